@@ -1,0 +1,150 @@
+"""Spin-flip throughput of the parallel-tempering engine on one MI355X.
+
+Workload (per step = one PT interval of 16 sweeps over the whole batch):
+  3D Edwards-Anderson +-J, L=16, 24 temperatures geometric in T = [0.2, 2.0],
+  2 replicas per sample, PT attempt every 16 sweeps (alternating even/odd
+  pairs), a measurement every 1000 sweeps, hot start from seeded Philox.
+  batched form: 2048 samples (32 groups of 64, one uint64 word per site) per
+  launch, the interval captured as one launch sequence;
+  plain form:   64 samples (one group), the per-group drop-in launches.
+value = batched spin-flip attempts per ns (whole job, inputs resident in HBM).
+
+Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "spin-glass_amd"))
+
+import numpy as np  # noqa: E402
+
+import sg  # noqa: E402
+
+L, T, R = 16, 24, 2
+T_MIN, T_MAX = 0.2, 2.0
+PT_EVERY, MEAS_EVERY = 16, 1000
+SAMPLES_BATCHED, SAMPLES_PLAIN = 2048, 64
+W_ALG = 1.22          # int32 VALU ops per spin-flip attempt (algorithmic)
+VALU_PEAK_TOPS = 39.3  # 256 CU x 64 lanes x 2.4 GHz, int32 ops
+HBM_PEAK_GBS = 8000.0
+
+
+def make(samples, form):
+    s = sg.Sim(L=L, n_samples=samples, n_replicas=R, T_range=(T_MIN, T_MAX, T),
+               coupling_seed=0x5EED0001, spin_seed=0x5EED0002, pt_every=PT_EVERY,
+               meas_every=MEAS_EVERY, form=form)
+    s.keep_records = False
+    s.init_lattice()
+    return s
+
+
+def time_form(samples, form, steps, warmup):
+    s = make(samples, form)
+    s.run(warmup * PT_EVERY)
+    s.set_profiling(True)
+    t0 = time.perf_counter()
+    s.run(steps * PT_EVERY)          # sg_run synchronises the stream on return
+    t1 = time.perf_counter()
+    n, ms, fl = s.kernel_stats(0)
+    flips = float(steps * PT_EVERY) * L ** 3 * 64 * s.G * R * T
+    out = dict(wall_s=t1 - t0, flips=flips, launches=n, sweep_kernel_ms=ms,
+               sweep_kernel_flips=fl)
+    s.close()
+    return out
+
+
+def cpu_baseline(budget_s=10.0):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, 16)
+    oracle.set_threads(cores)
+    betas = 1.0 / (T_MIN * (T_MAX / T_MIN) ** (np.arange(T) / (T - 1)))
+    o = oracle.OracleSim(L, 1, R, betas, pt_every=PT_EVERY, meas_every=MEAS_EVERY)
+    o.init()
+    o.run(PT_EVERY)
+    sweeps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        o.run(PT_EVERY)
+        sweeps += PT_EVERY
+    dt = time.perf_counter() - t0
+    flips = float(sweeps) * L ** 3 * 64 * R * T
+    return dict(value=flips / dt / 1e9, unit="spin flips/ns", cores=cores, kind="port",
+                sample=f"oracle/sg_oracle.c bit-sliced path, L=16, 64 samples x {T} T x {R} R, "
+                       f"{sweeps} sweeps with PT every {PT_EVERY}, {cores} OpenMP threads, "
+                       f"{dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (samples shard across GPUs with no "
+              "exchange; see DESIGN.md); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    b = time_form(SAMPLES_BATCHED, sg.SG_FORM_BATCHED, args.steps, args.warmup)
+    value = b["flips"] / b["wall_s"] / 1e9
+    plain = None
+    if not args.no_plain:
+        p = time_form(SAMPLES_PLAIN, sg.SG_FORM_PLAIN, args.steps, args.warmup)
+        plain = p["flips"] / p["wall_s"] / 1e9
+    avg_ms = b["sweep_kernel_ms"] / max(b["launches"], 1)
+    flips_per_launch = b["sweep_kernel_flips"] / max(b["launches"], 1)
+    achieved = W_ALG * flips_per_launch / (avg_ms * 1e-3) / 1e12
+    line = {
+        "metric": "spin flips/ns at L=16, batched vs plain",
+        "value": round(value, 3),
+        "unit": "spin flips/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(b["wall_s"] * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded Philox couplings and hot-start spins)",
+        "config": {
+            "workload": f"3D EA +-J PT, L={L}, {SAMPLES_BATCHED} samples (uint64 words), "
+                        f"{T} temperatures geometric in T=[{T_MIN},{T_MAX}], {R} replicas, "
+                        f"PT every {PT_EVERY} sweeps, measurement every {MEAS_EVERY}; "
+                        f"step = {PT_EVERY} sweeps",
+            "form": "batched",
+            "plain_flips_per_ns": None if plain is None else round(plain, 3),
+            "plain_workload": f"same, {SAMPLES_PLAIN} samples (one group), per-group launches",
+            "batched_over_plain": None if plain is None else round(value / plain, 3),
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_sweep_lds<16,false>",
+            "achieved": round(achieved, 3),
+            "peak": VALU_PEAK_TOPS,
+            "unit": "TOP/s (int32 VALU, W_alg = 1.22 ops per flip attempt)",
+            "frac": round(achieved / VALU_PEAK_TOPS, 4),
+            "avg_launch_ms": round(avg_ms, 4),
+            "flips_per_launch": flips_per_launch,
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+/*
+ * sg.h — C-ABI of libsg, the MI355X-native parallel-tempering engine for the
+ * 3D Edwards–Anderson ±J spin glass.
+ *
+ * The reference (Marco-Mendivil-Carboni/Spin-Glass) exposes its hot path
+ * through the C++ class `eamsim` (Program/src/eamsim.cuh:66-110) driven by
+ * `main` (Program/src/main.cu:8-84).  There is no FFI in the reference; each
+ * entry point below replaces one member of that class, as cited.  No C++
+ * exception and no torch type crosses this boundary: every call returns an
+ * sg_status and sg_last_error() gives the message (the reference throws
+ * `error`, Program/src/util.cuh:46-54, caught once in main.cu:70-75).
+ *
+ * Threading: one handle is used by one host thread at a time; different
+ * handles may run concurrently (each owns its own HIP stream).
+ *
+ * Canonical algorithm (the product and the CPU oracle under oracle/ both
+ * follow it bit for bit): 64 disorder samples per uint64 word (bit l =
+ * sample l of a "group"), spin bit 1 = +1, coupling bit 1 = J=-1; Philox4x32-10
+ * counters keyed by (group, replica, temperature slot, sweep, site); integer
+ * Metropolis thresholds floor(2^32 exp(-beta dE)); PT swaps of adjacent
+ * temperature slots, even pairs on even PT steps and odd pairs on odd steps.
+ */
+#ifndef SG_H
+#define SG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sg_status {
+  SG_OK = 0,
+  SG_ERR_INVALID = 1,   /* bad parameter or argument */
+  SG_ERR_DEVICE = 2,    /* HIP runtime error */
+  SG_ERR_NOMEM = 3,     /* allocation failed */
+  SG_ERR_FORMAT = 4,    /* checkpoint size / magic / checksum mismatch */
+  SG_ERR_STATE = 5      /* call out of order (e.g. run before init) */
+} sg_status;
+
+typedef struct sg_handle sg_handle;
+
+/* Run parameters.  They replace the reference's compile-time constants
+ * (Program/src/eamsim.cuh:13-25) and its constructor argument H
+ * (eamsim.cuh:73).  Zero-initialise, then fill; sg_params_default() fills the
+ * reference's own values (L=16, 2048 samples, 2 copies, 24 temperatures of
+ * the ladder beta_i = 2*2^(-4i/23), PT every 16, measurement every 1024). */
+typedef struct sg_params {
+  int32_t L;             /* even, 2..48 */
+  int32_t n_samples;     /* disorder samples, packed 64 per group */
+  int32_t n_temps;       /* temperature slots */
+  int32_t n_replicas;    /* real replicas per sample ("copies", >= 2 for q) */
+  const double *betas;   /* n_temps inverse temperatures, slot order; NULL: use T_min/T_max */
+  double T_min, T_max;   /* geometric temperature ladder (slot 0 = T_min) if betas == NULL */
+  double H;              /* external field, 0 <= H <= 4 (eamsim.cu:520) */
+  uint64_t coupling_seed;
+  uint64_t spin_seed;    /* initial spins, Metropolis and PT random numbers */
+  int32_t pt_every;      /* sweeps between PT steps (0 = no PT) */
+  int32_t meas_every;    /* sweeps between measurements (0 = none) */
+  int32_t device;        /* HIP device ordinal */
+  int32_t form;          /* SG_FORM_* */
+} sg_params;
+
+enum {
+  SG_FORM_PLAIN = 0,      /* per-interval launches, one group at a time */
+  SG_FORM_BATCHED = 1     /* all groups per launch, interval captured in a hipGraph */
+};
+
+/* Per-measurement callback: rec holds n_groups*n_temps*64 records of nfield
+ * int32 each (layout [group][temp][lane][field]); see sg_record_layout. */
+typedef void (*sg_obs_sink)(void *user, int64_t sweep, const int32_t *rec,
+                            int64_t n_records, int32_t nfield);
+
+/* replaces the reference's constants and ladder, eamsim.cuh:13-25 and
+ * eamsim::init_repib (eamsim.cu:725-744) */
+void sg_params_default(sg_params *p);
+
+/* replaces eamsim::eamsim(float H) (eamsim.cu:516-543): validates, allocates
+ * the device state, builds threshold tables */
+sg_status sg_create(const sg_params *p, sg_handle **out);
+
+/* replaces eamsim::~eamsim() (eamsim.cu:546-559) */
+sg_status sg_destroy(sg_handle *h);
+
+/* message of the last failed call on this handle (or of the last failed
+ * sg_create when h == NULL) */
+const char *sg_last_error(const sg_handle *h);
+
+/* replaces eamsim::init_lattice() (eamsim.cu:622-652): couplings from
+ * coupling_seed (shared by all replicas of a sample), hot-start spins from
+ * spin_seed; resets the sweep counter */
+sg_status sg_init_lattice(sg_handle *h);
+
+/* replaces eamsim::run_simulation(std::ofstream&) (eamsim.cu:679-722):
+ * n_sweeps sweeps with PT and measurements on the configured cadence; each
+ * measurement goes to sink (may be NULL) and into the handle's FP64
+ * accumulators */
+sg_status sg_run(sg_handle *h, int64_t n_sweeps, sg_obs_sink sink, void *user);
+
+/* replaces eamsim::save_checkpoint / load_checkpoint (eamsim.cu:655-676).
+ * The image carries a versioned header (L, sizes, ladder, seeds, sweep
+ * counter, accumulators) and a checksum; load rejects a size, magic,
+ * parameter or checksum mismatch.  save with buf == NULL returns the size. */
+sg_status sg_save_checkpoint(sg_handle *h, void *buf, size_t *size);
+sg_status sg_load_checkpoint(sg_handle *h, const void *buf, size_t size);
+
+/* --- inspection (no counterpart in the reference: used by the CLI, tests
+ *     and the analysis writer) ------------------------------------------- */
+
+int32_t sg_nfield(const sg_handle *h);            /* 2R + 1 + 3L */
+int32_t sg_n_groups(const sg_handle *h);
+int64_t sg_sweep_count(const sg_handle *h);
+sg_status sg_get_betas(const sg_handle *h, double *betas);            /* [T] */
+/* spins in natural site order s = x + L(y + Lz), [group][replica][temp][N] */
+sg_status sg_get_spins(sg_handle *h, uint64_t *out);
+sg_status sg_set_spins(sg_handle *h, const uint64_t *in);
+/* couplings [group][3][N], bit 1 = antiferromagnetic bond s -> s + e_d */
+sg_status sg_get_couplings(sg_handle *h, uint64_t *out);
+sg_status sg_set_couplings(sg_handle *h, const uint64_t *in);
+/* per-lane unsatisfied-bond count U and up-spin count K of every lattice
+ * [group][replica][temp][64] */
+sg_status sg_get_energies(sg_handle *h, int32_t *U, int32_t *K);
+/* accepted / attempted PT swaps per adjacent pair (t, t+1), summed over
+ * groups, replicas and lanes, [T-1] each */
+sg_status sg_get_swap_counts(sg_handle *h, uint64_t *accepted, uint64_t *attempted);
+/* FP64 per-temperature results over all measurements so far and all active
+ * samples: [T][SG_NRES] with the columns below */
+enum {
+  SG_RES_BETA = 0, SG_RES_E, SG_RES_M, SG_RES_Q, SG_RES_Q2, SG_RES_Q4,
+  SG_RES_BINDER, SG_RES_CHI0, SG_RES_CHI1, SG_RES_XI_L, SG_RES_NMEAS,
+  SG_NRES
+};
+sg_status sg_get_results(sg_handle *h, double *res);
+/* device time of sg_run calls since creation (ms, HIP events on the handle's
+ * stream) and the spin-flip attempts they performed */
+sg_status sg_get_timing(sg_handle *h, double *kernel_ms, double *flips);
+/* per-kernel timing: with profiling on, every sweep-kernel launch is
+ * bracketed by HIP events on the handle's stream (the batched form then
+ * launches directly instead of through its hipGraph).  kind 0 = sweep kernel,
+ * 1 = PT exchange, 2 = measurement.  Returns launches, summed device ms and
+ * the spin-flip attempts of those launches. */
+sg_status sg_set_profiling(sg_handle *h, int32_t on);
+sg_status sg_get_kernel_stats(sg_handle *h, int32_t kind, int64_t *launches,
+                              double *total_ms, double *flips);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,48 @@
+// sg_kernels.h — host-side launch entry points of the HIP kernels of libsg.
+#pragma once
+
+#include "sg_common.h"
+
+namespace sg {
+
+// Per-job device state read by the kernels, so that a launch sequence can be
+// captured once in a hipGraph and replayed while the counters advance.
+struct JobState {
+  uint32_t sweep;     // sweeps completed
+  uint32_t pt_every;  // PT cadence (0 = none)
+  uint32_t pad[2];
+};
+
+// largest L whose lattice the LDS-resident sweep kernel holds
+bool lds_kernel_supported(int L);
+
+// k sweeps of every lattice in descs (one work-group per lattice); with
+// want_energy the per-lane U and K of the final state go to the energy slots.
+hipError_t launch_sweep(int L, bool field, uint64_t *spins, const uint64_t *J,
+                        const LatDesc *descs, int ndesc, const ThrRec *thr,
+                        const JobState *jobs, int nsweeps, int want_energy,
+                        int32_t *energy, hipStream_t st);
+
+// energy of every lattice (no sweep)
+hipError_t launch_energy(int L, uint64_t *spins, const uint64_t *J,
+                         const LatDesc *descs, int ndesc, int32_t *energy,
+                         hipStream_t st);
+
+// PT step of every pair whose parity matches the job's current step; pairs
+// of the other parity return at once.
+hipError_t launch_pt(uint64_t *spins, int32_t *energy, const PairDesc *pairs,
+                     int npairs, const JobState *jobs,
+                     unsigned long long *counters, hipStream_t st);
+
+hipError_t launch_measure(int L, int R, const uint64_t *spins, const uint64_t *J,
+                          const MeasDesc *md, int nmd, int32_t *out, int nfield,
+                          hipStream_t st);
+
+hipError_t launch_init_spins(int L, uint64_t *spins, const LatDesc *descs,
+                             int ndesc, hipStream_t st);
+hipError_t launch_init_couplings(int L, uint64_t *J, uint64_t j_off, uint32_t g,
+                                 uint32_t key0, uint32_t key1, hipStream_t st);
+
+hipError_t launch_advance(JobState *jobs, int njobs, int nsweeps, hipStream_t st);
+
+}  // namespace sg

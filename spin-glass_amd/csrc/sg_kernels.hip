@@ -1,0 +1,538 @@
+// sg_kernels.hip — CDNA4 (gfx950) kernels of libsg.
+//
+// Hot path: checkerboard Metropolis sweeps on multispin words (64 samples per
+// uint64), with the lattice resident in LDS for the whole launch and the
+// couplings of each thread's sites held in VGPRs.  The local-field count is
+// a bit-sliced full-adder tree and the accept decision a comparison of one
+// Philox word per site against integer thresholds, so no float enters the
+// update.  Reference counterpart: perform_MC_steps, Program/src/eamsim.cu:
+// 112-208 (one curand() and one LUT read per spin there).
+#include "sg_kernels.h"
+
+namespace sg {
+
+__device__ __forceinline__ uint64_t maj3(uint64_t a, uint64_t b, uint64_t c) {
+  return (a & b) | (c & (a | b));
+}
+
+template <int L>
+struct Geo {
+  static constexpr int N = L * L * L;
+  static constexpr int NH = N / 2;
+  static constexpr int Q = N / 4;          // Philox quads: one per thread per sweep
+  static constexpr int LH = L / 2;
+  static constexpr int NT = ((Q + 63) / 64) * 64;
+  static constexpr int NW = NT / 64;
+};
+
+// word index of (x, y, z) in the checkerboard-split layout
+template <int L>
+__device__ __forceinline__ int cb_index(int x, int y, int z) {
+  return ((x + y + z) & 1) * Geo<L>::NH + (x >> 1) + Geo<L>::LH * (y + L * z);
+}
+
+// Bit-sliced add: accumulator planes A (NP planes) += 3-bit count (u2 u1 u0).
+template <int NP>
+__device__ __forceinline__ void vc_add3(uint64_t (&A)[NP], uint64_t u0, uint64_t u1,
+                                        uint64_t u2) {
+  uint64_t c = A[0] & u0;
+  A[0] ^= u0;
+  uint64_t s = A[1] ^ u1 ^ c;
+  c = maj3(A[1], u1, c);
+  A[1] = s;
+  s = A[2] ^ u2 ^ c;
+  c = maj3(A[2], u2, c);
+  A[2] = s;
+#pragma unroll
+  for (int p = 3; p < NP; ++p) {
+    const uint64_t t = A[p] & c;
+    A[p] ^= c;
+    c = t;
+  }
+}
+
+template <int NP>
+__device__ __forceinline__ void vc_add1(uint64_t (&A)[NP], uint64_t w) {
+  uint64_t c = w;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const uint64_t t = A[p] & c;
+    A[p] ^= c;
+    c = t;
+  }
+}
+
+// Sum the bit-sliced per-thread counters of a work-group into one integer per
+// lane (sample).  A holds P0 planes on entry; the butterfly adds lanes l and
+// l^2^k (6 levels, +1 plane each) so every lane holds its wave's total; the
+// per-wave totals then meet in wave 0 through LDS and a second 6-level
+// butterfly (lanes >= NW contribute zero), after which every lane of wave 0
+// holds the work-group total and lane l extracts sample l.  Result valid in
+// wave 0.  red must hold NW*24 words.  Every thread must call it.
+template <int P0, int NW>
+__device__ int32_t wg_sum_planes(const uint64_t (&A0)[P0], uint64_t *red) {
+  constexpr int PW = P0 + 6, PT = P0 + 12;
+  uint64_t A[PT];
+#pragma unroll
+  for (int p = 0; p < P0; ++p) A[p] = A0[p];
+#pragma unroll
+  for (int lvl = 0; lvl < 6; ++lvl) {
+    const int np = P0 + lvl;
+    uint64_t carry = 0;
+#pragma unroll
+    for (int p = 0; p < np; ++p) {
+      const uint64_t b = __shfl_xor(A[p], 1 << lvl);
+      const uint64_t a = A[p];
+      A[p] = a ^ b ^ carry;
+      carry = maj3(a, b, carry);
+    }
+    A[np] = carry;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int p = 0; p < PW; ++p) red[wave * 24 + p] = A[p];
+  }
+  __syncthreads();
+  int32_t v = 0;
+  if (wave == 0) {
+#pragma unroll
+    for (int p = 0; p < PW; ++p) A[p] = lane < NW ? red[lane * 24 + p] : 0;
+#pragma unroll
+    for (int lvl = 0; lvl < 6; ++lvl) {
+      const int np = PW + lvl;
+      uint64_t carry = 0;
+#pragma unroll
+      for (int p = 0; p < np; ++p) {
+        const uint64_t b = __shfl_xor(A[p], 1 << lvl);
+        const uint64_t a = A[p];
+        A[p] = a ^ b ^ carry;
+        carry = maj3(a, b, carry);
+      }
+      A[np] = carry;
+    }
+#pragma unroll
+    for (int p = 0; p < PT; ++p) v |= (int32_t)((A[p] >> lane) & 1) << p;
+  }
+  __syncthreads();
+  return v;
+}
+
+// lanes whose bit-sliced count (u2 u1 u0) is >= k (k in 0..7, per thread)
+__device__ __forceinline__ uint64_t count_ge(uint64_t u2, uint64_t u1, uint64_t u0, int k) {
+  const uint64_t k2 = 0 - (uint64_t)((k >> 2) & 1);
+  const uint64_t k1 = 0 - (uint64_t)((k >> 1) & 1);
+  const uint64_t k0 = 0 - (uint64_t)(k & 1);
+  uint64_t gt = u2 & ~k2;
+  uint64_t eq = ~(u2 ^ k2);
+  gt |= eq & u1 & ~k1;
+  eq &= ~(u1 ^ k1);
+  gt |= eq & u0 & ~k0;
+  eq &= ~(u0 ^ k0);
+  return gt | eq;
+}
+
+// ------------------------------------------------------------- sweep kernel
+// One work-group per lattice.  Thread j owns the Philox quad j: black and
+// white sites at h = j and h = j + N/4.  Per sweep it draws one Philox block
+// (r0, r1 for its black sites, r2, r3 for its white sites).
+template <int L, bool FIELD>
+__global__ __launch_bounds__(Geo<L>::NT) void k_sweep_lds(
+    uint64_t *__restrict__ spins, const uint64_t *__restrict__ Jg,
+    const LatDesc *__restrict__ descs, const ThrRec *__restrict__ thr,
+    const JobState *__restrict__ jobs, int nsweeps, int want_energy,
+    int32_t *__restrict__ energy) {
+  using G = Geo<L>;
+  __shared__ uint64_t lat[G::N];
+  __shared__ uint64_t red[G::NW * 24];
+
+  const LatDesc d = descs[blockIdx.x];
+  const int tid = threadIdx.x;
+  const bool active = tid < G::Q;
+  uint64_t *gl = spins + d.spin_off;
+  for (int i = tid; i < G::N; i += G::NT) lat[i] = gl[i];
+
+  const uint64_t *Jx = Jg + d.j_off, *Jy = Jx + G::N, *Jz = Jy + G::N;
+  int nb[2][2][6];
+  uint64_t jw[2][2][6];
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int h = tid + q * G::Q;
+      const int xh = h % G::LH, y = (h / G::LH) % L, z = h / (G::LH * L);
+      const int yp = (y + 1) % L, ym = (y + L - 1) % L;
+      const int zp = (z + 1) % L, zm = (z + L - 1) % L;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int x = 2 * xh + ((y + z + c) & 1);
+        const int xp = (x + 1) % L, xm = (x + L - 1) % L;
+        nb[q][c][0] = cb_index<L>(xp, y, z);
+        nb[q][c][1] = cb_index<L>(xm, y, z);
+        nb[q][c][2] = cb_index<L>(x, yp, z);
+        nb[q][c][3] = cb_index<L>(x, ym, z);
+        nb[q][c][4] = cb_index<L>(x, y, zp);
+        nb[q][c][5] = cb_index<L>(x, y, zm);
+        const int s = x + L * (y + L * z);
+        jw[q][c][0] = Jx[s];
+        jw[q][c][1] = Jx[xm + L * (y + L * z)];
+        jw[q][c][2] = Jy[s];
+        jw[q][c][3] = Jy[x + L * (ym + L * z)];
+        jw[q][c][4] = Jz[s];
+        jw[q][c][5] = Jz[x + L * (y + L * zm)];
+      }
+    }
+  }
+  ThrRec tr{};
+  uint32_t n0 = 0;
+  if (nsweeps > 0) {  // an energy-only launch passes no tables
+    tr = thr[d.thr_idx];
+    n0 = jobs[d.job].sweep;
+  }
+  const uint32_t sw3 = (kTagMC << 28) | (d.rt & 0x0FFFFFFFu);
+  __syncthreads();
+
+  for (int it = 0; it < nsweeps; ++it) {
+    const u32x4 r = philox((uint32_t)tid, n0 + (uint32_t)it, d.g, sw3, d.key0, d.key1);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (active) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int self = c * G::NH + tid + q * G::Q;
+          const uint64_t s = lat[self];
+          const uint64_t b0 = s ^ lat[nb[q][c][0]] ^ jw[q][c][0];
+          const uint64_t b1 = s ^ lat[nb[q][c][1]] ^ jw[q][c][1];
+          const uint64_t b2 = s ^ lat[nb[q][c][2]] ^ jw[q][c][2];
+          const uint64_t b3 = s ^ lat[nb[q][c][3]] ^ jw[q][c][3];
+          const uint64_t b4 = s ^ lat[nb[q][c][4]] ^ jw[q][c][4];
+          const uint64_t b5 = s ^ lat[nb[q][c][5]] ^ jw[q][c][5];
+          const uint64_t s1 = b0 ^ b1 ^ b2, c1 = maj3(b0, b1, b2);
+          const uint64_t s2 = b3 ^ b4 ^ b5, c2 = maj3(b3, b4, b5);
+          const uint64_t u0 = s1 ^ s2, c3 = s1 & s2;
+          const uint64_t u1 = c1 ^ c2 ^ c3, u2 = maj3(c1, c2, c3);
+          const uint32_t rr = r.v[2 * c + q];
+          uint64_t f;
+          if (!FIELD) {
+            // u >= 3 always flips; u = 2, 1, 0 flip iff rr < T(u)
+            const uint64_t R0 = rr < tr.t[0][0] ? ~0ull : 0ull;
+            const uint64_t R1 = rr < tr.t[0][1] ? ~0ull : 0ull;
+            const uint64_t R2 = rr < tr.t[0][2] ? ~0ull : 0ull;
+            const uint64_t lo = (u0 & R1) | (~u0 & R0);
+            const uint64_t hi = u0 | R2;
+            f = u2 | (u1 & hi) | (~u1 & lo);
+          } else {
+            int k0 = 0, k1 = 0;
+#pragma unroll
+            for (int u = 0; u < 7; ++u) {
+              k0 += ((uint32_t)u < tr.nfin[0]) & (rr >= tr.t[0][u]);
+              k1 += ((uint32_t)u < tr.nfin[1]) & (rr >= tr.t[1][u]);
+            }
+            const uint64_t f0 = count_ge(u2, u1, u0, k0);
+            const uint64_t f1 = count_ge(u2, u1, u0, k1);
+            f = (s & f1) | (~s & f0);
+          }
+          lat[self] = s ^ f;
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  if (want_energy) {
+    // every bond touches exactly one black site (colour 0)
+    uint64_t AU[4] = {0, 0, 0, 0};
+    uint64_t AK[3] = {0, 0, 0};
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int self = tid + q * G::Q;
+        const uint64_t s = lat[self];
+        const uint64_t b0 = s ^ lat[nb[q][0][0]] ^ jw[q][0][0];
+        const uint64_t b1 = s ^ lat[nb[q][0][1]] ^ jw[q][0][1];
+        const uint64_t b2 = s ^ lat[nb[q][0][2]] ^ jw[q][0][2];
+        const uint64_t b3 = s ^ lat[nb[q][0][3]] ^ jw[q][0][3];
+        const uint64_t b4 = s ^ lat[nb[q][0][4]] ^ jw[q][0][4];
+        const uint64_t b5 = s ^ lat[nb[q][0][5]] ^ jw[q][0][5];
+        const uint64_t s1 = b0 ^ b1 ^ b2, c1 = maj3(b0, b1, b2);
+        const uint64_t s2 = b3 ^ b4 ^ b5, c2 = maj3(b3, b4, b5);
+        const uint64_t u0 = s1 ^ s2, c3 = s1 & s2;
+        const uint64_t u1 = c1 ^ c2 ^ c3, u2 = maj3(c1, c2, c3);
+        vc_add3(AU, u0, u1, u2);
+        vc_add1(AK, s);
+        vc_add1(AK, lat[G::NH + self]);
+      }
+    }
+    const int32_t U = wg_sum_planes<4, G::NW>(AU, red);
+    const int32_t K = wg_sum_planes<3, G::NW>(AK, red);
+    if (tid < 64) {
+      energy[(size_t)d.e_idx * 128 + tid] = U;
+      energy[(size_t)d.e_idx * 128 + 64 + tid] = K;
+    }
+  }
+
+  for (int i = tid; i < G::N; i += G::NT) gl[i] = lat[i];
+}
+
+// --------------------------------------------------------------- PT kernel
+// One work-group per adjacent pair.  Wave 0 decides the 64 lanes (one
+// Philox word each), the ballot is the exchange mask, and the whole group
+// exchanges the masked bits of the two lattices.  Reference counterpart:
+// shuffle, eamsim.cu:43-90 (which swaps beta labels instead of spins).
+__global__ __launch_bounds__(256) void k_pt(uint64_t *__restrict__ spins,
+                                            int32_t *__restrict__ energy,
+                                            const PairDesc *__restrict__ pairs,
+                                            const JobState *__restrict__ jobs,
+                                            unsigned long long *__restrict__ counters) {
+  const PairDesc p = pairs[blockIdx.x];
+  const JobState js = jobs[p.job];
+  if (js.pt_every == 0) return;
+  const uint32_t b = js.sweep / js.pt_every;
+  if ((b & 1u) != p.parity) return;
+  __shared__ unsigned long long smask;
+  const int tid = threadIdx.x;
+  if (tid < 64) {
+    const int l = tid;
+    const uint32_t sw3 = (kTagPT << 28) | (p.rt & 0x0FFFFFFFu);
+    const u32x4 r = philox((uint32_t)(l >> 2), b, p.g, sw3, p.key0, p.key1);
+    const uint32_t rr = r.v[l & 3];
+    int32_t *Ea = energy + (size_t)p.e_a * 128, *Eb = energy + (size_t)p.e_b * 128;
+    const int32_t ua = Ea[l], ub = Eb[l], ka = Ea[64 + l], kb = Eb[64 + l];
+    const bool acc = pt_accept(p.beta_a, p.beta_b, ua, ub, ka, kb, p.H, rr);
+    const unsigned long long m = __ballot(acc);
+    if (acc) {
+      Ea[l] = ub; Eb[l] = ua;
+      Ea[64 + l] = kb; Eb[64 + l] = ka;
+    }
+    if (l == 0) {
+      smask = m;
+      atomicAdd(&counters[p.cnt_idx], (unsigned long long)__popcll(m));
+    }
+  }
+  __syncthreads();
+  const uint64_t m = smask;
+  if (m == 0) return;
+  uint64_t *A = spins + p.a_off, *B = spins + p.b_off;
+  for (uint32_t i = tid; i < p.n_words; i += 256) {
+    const uint64_t a = A[i], bb = B[i];
+    const uint64_t dd = (a ^ bb) & m;
+    A[i] = a ^ dd;
+    B[i] = bb ^ dd;
+  }
+}
+
+// ---------------------------------------------------------- measure kernel
+// One work-group (4 waves) per (group, temperature).  Lane l of every wave
+// is sample l; each wave walks a quarter of the (y, z) rows with wave-uniform
+// (scalar) loads and counts, per lane, U and K of every replica and the
+// overlap W with its x-, y- and z-plane profiles.  Reference counterpart:
+// compute_q (eamsim.cu:451-511) plus the E, M scan of perform_PT_shuffle.
+template <int L>
+__global__ __launch_bounds__(256) void k_measure(const uint64_t *__restrict__ spins,
+                                                 const uint64_t *__restrict__ Jg,
+                                                 const MeasDesc *__restrict__ md, int R,
+                                                 int32_t *__restrict__ out, int nfield) {
+  constexpr int N = L * L * L;
+  __shared__ int32_t sx[L][64], sy[L][64], sz[L][64];
+  __shared__ int32_t sU[8][64], sK[8][64], sW[64];
+  const MeasDesc d = md[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < L * 64; i += 256) {
+    (&sx[0][0])[i] = 0; (&sy[0][0])[i] = 0; (&sz[0][0])[i] = 0;
+  }
+  for (int i = tid; i < 8 * 64; i += 256) { (&sU[0][0])[i] = 0; (&sK[0][0])[i] = 0; }
+  if (tid < 64) sW[tid] = 0;
+  __syncthreads();
+  const uint64_t *Jx = Jg + d.j_off, *Jy = Jx + N, *Jz = Jy + N;
+  for (int rho = 0; rho < R; ++rho) {
+    const uint64_t *S = spins + d.spin_off + (uint64_t)rho * d.rho_stride;
+    int32_t u = 0, k = 0;
+    for (int row = wave; row < L * L; row += 4) {
+      const int y = row % L, z = row / L;
+      const int yp = (y + 1) % L, zp = (z + 1) % L;
+#pragma unroll 4
+      for (int x = 0; x < L; ++x) {
+        const int xp = (x + 1) % L;
+        const int sn = x + L * (y + L * z);
+        const uint64_t s = S[cb_index<L>(x, y, z)];
+        const uint64_t bx = s ^ S[cb_index<L>(xp, y, z)] ^ Jx[sn];
+        const uint64_t by = s ^ S[cb_index<L>(x, yp, z)] ^ Jy[sn];
+        const uint64_t bz = s ^ S[cb_index<L>(x, y, zp)] ^ Jz[sn];
+        u += (int32_t)((bx >> lane) & 1) + (int32_t)((by >> lane) & 1) +
+             (int32_t)((bz >> lane) & 1);
+        k += (int32_t)((s >> lane) & 1);
+      }
+    }
+    atomicAdd(&sU[rho][lane], u);
+    atomicAdd(&sK[rho][lane], k);
+  }
+  if (R >= 2) {
+    const uint64_t *S0 = spins + d.spin_off, *S1 = S0 + d.rho_stride;
+    int32_t wx[L];
+#pragma unroll
+    for (int x = 0; x < L; ++x) wx[x] = 0;
+    int32_t wt = 0;
+    for (int row = wave; row < L * L; row += 4) {
+      const int y = row % L, z = row / L;
+      int32_t rs = 0;
+#pragma unroll
+      for (int x = 0; x < L; ++x) {
+        const int i = cb_index<L>(x, y, z);
+        const int32_t wb = (int32_t)(((S0[i] ^ S1[i]) >> lane) & 1);
+        wx[x] += wb;
+        rs += wb;
+      }
+      atomicAdd(&sy[y][lane], rs);
+      atomicAdd(&sz[z][lane], rs);
+      wt += rs;
+    }
+#pragma unroll
+    for (int x = 0; x < L; ++x) atomicAdd(&sx[x][lane], wx[x]);
+    atomicAdd(&sW[lane], wt);
+  }
+  __syncthreads();
+  int32_t *o = out + d.out_off;
+  for (int i = tid; i < 64 * nfield; i += 256) {
+    const int l = i / nfield, f = i % nfield;
+    int32_t v;
+    if (f < R) v = sU[f][l];
+    else if (f < 2 * R) v = sK[f - R][l];
+    else if (f == 2 * R) v = R >= 2 ? sW[l] : 0;
+    else {
+      const int g = f - 2 * R - 1;
+      const int dir = g / L, c = g % L;
+      v = R < 2 ? 0 : (dir == 0 ? sx[c][l] : dir == 1 ? sy[c][l] : sz[c][l]);
+    }
+    o[i] = v;
+  }
+}
+
+// ------------------------------------------------------------ init kernels
+template <int L>
+__global__ __launch_bounds__(256) void k_init_spins(uint64_t *__restrict__ spins,
+                                                    const LatDesc *__restrict__ descs) {
+  constexpr int N = L * L * L;
+  const LatDesc d = descs[blockIdx.y];
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= N) return;
+  const int x = s % L, y = (s / L) % L, z = s / (L * L);
+  const uint32_t sw3 = (kTagSpin << 28) | (d.rt & 0x0FFFFFFFu);
+  const u32x4 r = philox((uint32_t)s, 0u, d.g, sw3, d.key0, d.key1);
+  spins[d.spin_off + cb_index<L>(x, y, z)] = (uint64_t)r.v[0] | ((uint64_t)r.v[1] << 32);
+}
+
+__global__ __launch_bounds__(256) void k_init_couplings(uint64_t *__restrict__ J, uint64_t j_off,
+                                                        int N, uint32_t g, uint32_t key0,
+                                                        uint32_t key1) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  const int dir = blockIdx.y;
+  if (s >= N) return;
+  const u32x4 r = philox((uint32_t)s, (uint32_t)dir, g, kTagCoup << 28, key0, key1);
+  J[j_off + (uint64_t)dir * N + s] = (uint64_t)r.v[0] | ((uint64_t)r.v[1] << 32);
+}
+
+__global__ void k_advance(JobState *jobs, int njobs, int nsweeps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < njobs) jobs[i].sweep += (uint32_t)nsweeps;
+}
+
+// ---------------------------------------------------------------- dispatch
+#define SG_LDS_SIZES(X) X(2) X(4) X(6) X(8) X(10) X(12) X(14) X(16)
+#define SG_ALL_SIZES(X) SG_LDS_SIZES(X) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) \
+  X(34) X(36) X(38) X(40) X(42) X(44) X(46) X(48)
+
+bool lds_kernel_supported(int L) {
+  switch (L) {
+#define X(l) case l: return true;
+    SG_LDS_SIZES(X)
+#undef X
+    default: return false;
+  }
+}
+
+hipError_t launch_sweep(int L, bool field, uint64_t *spins, const uint64_t *J,
+                        const LatDesc *descs, int ndesc, const ThrRec *thr,
+                        const JobState *jobs, int nsweeps, int want_energy,
+                        int32_t *energy, hipStream_t st) {
+  if (ndesc <= 0) return hipSuccess;
+  switch (L) {
+#define X(l)                                                                              \
+  case l:                                                                                 \
+    if (field)                                                                            \
+      hipLaunchKernelGGL((k_sweep_lds<l, true>), dim3(ndesc), dim3(Geo<l>::NT), 0, st,    \
+                         spins, J, descs, thr, jobs, nsweeps, want_energy, energy);       \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_sweep_lds<l, false>), dim3(ndesc), dim3(Geo<l>::NT), 0, st,   \
+                         spins, J, descs, thr, jobs, nsweeps, want_energy, energy);       \
+    return hipGetLastError();
+    SG_LDS_SIZES(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_energy(int L, uint64_t *spins, const uint64_t *J, const LatDesc *descs,
+                         int ndesc, int32_t *energy, hipStream_t st) {
+  return launch_sweep(L, false, spins, J, descs, ndesc, nullptr, nullptr, 0, 1, energy, st);
+}
+
+hipError_t launch_pt(uint64_t *spins, int32_t *energy, const PairDesc *pairs, int npairs,
+                     const JobState *jobs, unsigned long long *counters, hipStream_t st) {
+  if (npairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pt, dim3(npairs), dim3(256), 0, st, spins, energy, pairs, jobs,
+                     counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_measure(int L, int R, const uint64_t *spins, const uint64_t *J,
+                          const MeasDesc *md, int nmd, int32_t *out, int nfield,
+                          hipStream_t st) {
+  if (nmd <= 0) return hipSuccess;
+  if (R < 1 || R > 8) return hipErrorInvalidValue;
+  switch (L) {
+#define X(l)                                                                            \
+  case l:                                                                               \
+    hipLaunchKernelGGL(k_measure<l>, dim3(nmd), dim3(256), 0, st, spins, J, md, R, out, \
+                       nfield);                                                         \
+    return hipGetLastError();
+    SG_ALL_SIZES(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_init_spins(int L, uint64_t *spins, const LatDesc *descs, int ndesc,
+                             hipStream_t st) {
+  if (ndesc <= 0) return hipSuccess;
+  const int N = L * L * L;
+  switch (L) {
+#define X(l)                                                                           \
+  case l:                                                                              \
+    hipLaunchKernelGGL(k_init_spins<l>, dim3((N + 255) / 256, ndesc), dim3(256), 0, st, \
+                       spins, descs);                                                  \
+    return hipGetLastError();
+    SG_ALL_SIZES(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_init_couplings(int L, uint64_t *J, uint64_t j_off, uint32_t g,
+                                 uint32_t key0, uint32_t key1, hipStream_t st) {
+  const int N = L * L * L;
+  hipLaunchKernelGGL(k_init_couplings, dim3((N + 255) / 256, 3), dim3(256), 0, st, J, j_off,
+                     N, g, key0, key1);
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(JobState *jobs, int njobs, int nsweeps, hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_advance, dim3((njobs + 255) / 256), dim3(256), 0, st, jobs, njobs,
+                     nsweeps);
+  return hipGetLastError();
+}
+
+}  // namespace sg

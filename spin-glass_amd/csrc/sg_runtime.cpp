@@ -1,0 +1,793 @@
+// sg_runtime.cpp — host runtime and C-ABI of libsg (include/sg.h).
+//
+// Mirrors the reference's host object `eamsim` (Program/src/eamsim.cuh:66-110,
+// eamsim.cu:513-744): construction validates H and allocates, init_lattice
+// draws couplings and spins, run_simulation loops over measurement intervals,
+// save/load_checkpoint serialise the state.  Differences: every size is a
+// run-time parameter, the random numbers are counter-based (runs are
+// reproducible and resumable bit for bit), and errors are status codes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sg.h"
+#include "sg_common.h"
+#include "sg_kernels.h"
+
+using namespace sg;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct SampleAcc {         // per (sample, temperature)
+  int64_t n = 0;
+  double sum_e = 0, sum_m = 0, sum_absq = 0, sum_q1sq = 0;
+  int64_t sum_Q = 0;
+  unsigned __int128 sum_Q2 = 0, sum_Q4 = 0;
+};
+
+struct Fail {
+  sg_status code;
+  std::string msg;
+};
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      throw Fail{SG_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)};  \
+  } while (0)
+
+template <class T>
+void dev_alloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    throw Fail{SG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)};
+  }
+}
+
+uint64_t fnv1a(const void *data, size_t n, uint64_t h = 1469598103934665603ull) {
+  const unsigned char *p = (const unsigned char *)data;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= p[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+}  // namespace
+
+struct sg_handle {
+  std::string err;
+  int L = 0, N = 0, G = 0, R = 0, T = 0, S = 0, pt_every = 0, meas_every = 0, form = 0;
+  int device = 0;
+  double H = 0;
+  std::vector<double> beta;
+  uint64_t cseed = 0, sseed = 0;
+  bool field = false;
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  uint64_t *d_spins = nullptr, *d_J = nullptr;
+  ThrRec *d_thr = nullptr;
+  int32_t *d_energy = nullptr, *d_rec = nullptr;
+  LatDesc *d_lat = nullptr;
+  PairDesc *d_pairs = nullptr;
+  MeasDesc *d_meas = nullptr;
+  JobState *d_jobs = nullptr;
+  unsigned long long *d_cnt = nullptr;
+  int npairs = 0, nfield = 0;
+
+  std::vector<int32_t> h_rec;
+  std::vector<SampleAcc> acc;     // [sample][t]
+  std::vector<uint64_t> swap_att; // [T-1]
+  std::vector<double> cosw, sinw; // [L]
+  bool initialized = false, energy_valid = false;
+  int64_t sweep = 0, n_meas = 0;
+  double kernel_ms = 0, flips = 0;
+
+  // per-kernel profiling (sg_set_profiling)
+  struct KStat { int64_t launches = 0; double ms = 0, flips = 0; };
+  struct Pending { hipEvent_t a, b; int kind; double flips; };
+  bool profiling = false;
+  KStat kstat[3];
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> ev_pool;
+
+  hipEvent_t take_event() {
+    if (!ev_pool.empty()) {
+      hipEvent_t e = ev_pool.back();
+      ev_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+  template <class F>
+  void timed(int kind, double fl, F &&launch) {
+    if (!profiling) {
+      launch();
+      return;
+    }
+    Pending p{take_event(), take_event(), kind, fl};
+    HIPCHK(hipEventRecord(p.a, st));
+    launch();
+    HIPCHK(hipEventRecord(p.b, st));
+    pending.push_back(p);
+  }
+  void collect_profile() {
+    if (pending.empty()) return;
+    HIPCHK(hipStreamSynchronize(st));
+    for (auto &p : pending) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+      kstat[p.kind].launches += 1;
+      kstat[p.kind].ms += ms;
+      kstat[p.kind].flips += p.flips;
+      ev_pool.push_back(p.a);
+      ev_pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+
+  hipGraphExec_t graph = nullptr;   // batched form: one PT interval
+  int graph_seg = -1;
+  bool graph_energy = false;
+
+  ~sg_handle() { release(); }
+
+  void release() {
+    if (graph) hipGraphExecDestroy(graph);
+    graph = nullptr;
+    hipFree(d_spins); hipFree(d_J); hipFree(d_thr); hipFree(d_energy); hipFree(d_rec);
+    hipFree(d_lat); hipFree(d_pairs); hipFree(d_meas); hipFree(d_jobs); hipFree(d_cnt);
+    d_spins = d_J = nullptr; d_thr = nullptr; d_energy = d_rec = nullptr; d_lat = nullptr;
+    d_pairs = nullptr; d_meas = nullptr; d_jobs = nullptr; d_cnt = nullptr;
+    for (auto &p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    pending.clear();
+    for (auto e : ev_pool) hipEventDestroy(e);
+    ev_pool.clear();
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    if (st) hipStreamDestroy(st);
+    ev0 = ev1 = nullptr; st = nullptr;
+  }
+
+  size_t lat_index(int g, int rho, int t) const { return ((size_t)g * R + rho) * T + t; }
+  size_t n_lat() const { return (size_t)G * R * T; }
+
+  // ------------------------------------------------------------ set-up
+  void create(const sg_params &p) {
+    L = p.L; G = (p.n_samples + 63) / 64; S = p.n_samples; R = p.n_replicas; T = p.n_temps;
+    N = L * L * L; H = p.H; cseed = p.coupling_seed; sseed = p.spin_seed;
+    pt_every = p.pt_every; meas_every = p.meas_every; form = p.form; device = p.device;
+    field = H != 0.0;
+    beta.resize(T);
+    if (p.betas) {
+      for (int t = 0; t < T; ++t) beta[t] = p.betas[t];
+    } else {
+      for (int t = 0; t < T; ++t) {
+        const double f = T > 1 ? (double)t / (double)(T - 1) : 0.0;
+        beta[t] = 1.0 / (p.T_min * std::pow(p.T_max / p.T_min, f));
+      }
+    }
+    for (int t = 0; t < T; ++t)
+      if (!(beta[t] > 0.0) || !std::isfinite(beta[t]))
+        throw Fail{SG_ERR_INVALID, "every beta must be positive and finite"};
+
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    nfield = 2 * R + 1 + 3 * L;
+
+    dev_alloc(&d_spins, n_lat() * N);
+    dev_alloc(&d_J, (size_t)G * 3 * N);
+    dev_alloc(&d_thr, (size_t)T);
+    dev_alloc(&d_energy, n_lat() * 128);
+    dev_alloc(&d_rec, (size_t)G * T * 64 * nfield);
+    dev_alloc(&d_lat, n_lat());
+    dev_alloc(&d_meas, (size_t)G * T);
+    dev_alloc(&d_jobs, 1);
+    dev_alloc(&d_cnt, (size_t)std::max(T - 1, 1));
+
+    // thresholds
+    std::vector<ThrRec> thr(T);
+    for (int t = 0; t < T; ++t) build_thresholds(beta[t], thr[t]);
+    HIPCHK(hipMemcpy(d_thr, thr.data(), sizeof(ThrRec) * T, hipMemcpyHostToDevice));
+
+    // lattice descriptors [g][rho][t]
+    std::vector<LatDesc> ld(n_lat());
+    for (int g = 0; g < G; ++g)
+      for (int rho = 0; rho < R; ++rho)
+        for (int t = 0; t < T; ++t) {
+          const size_t i = lat_index(g, rho, t);
+          LatDesc &d = ld[i];
+          std::memset(&d, 0, sizeof(d));
+          d.spin_off = i * (uint64_t)N;
+          d.j_off = (uint64_t)g * 3 * N;
+          d.key0 = (uint32_t)sseed; d.key1 = (uint32_t)(sseed >> 32);
+          d.g = (uint32_t)g;
+          d.rt = ((uint32_t)rho << 16) | (uint32_t)t;
+          d.thr_idx = (uint32_t)t;
+          d.e_idx = (uint32_t)i;
+          d.job = 0;
+        }
+    HIPCHK(hipMemcpy(d_lat, ld.data(), sizeof(LatDesc) * ld.size(), hipMemcpyHostToDevice));
+
+    // PT pairs [g][rho][t], t = 0..T-2
+    npairs = G * R * std::max(T - 1, 0);
+    dev_alloc(&d_pairs, (size_t)npairs);
+    std::vector<PairDesc> pd;
+    pd.reserve(npairs);
+    for (int g = 0; g < G; ++g)
+      for (int rho = 0; rho < R; ++rho)
+        for (int t = 0; t + 1 < T; ++t) {
+          PairDesc d;
+          std::memset(&d, 0, sizeof(d));
+          d.a_off = lat_index(g, rho, t) * (uint64_t)N;
+          d.b_off = lat_index(g, rho, t + 1) * (uint64_t)N;
+          d.beta_a = beta[t]; d.beta_b = beta[t + 1]; d.H = H;
+          d.e_a = (uint32_t)lat_index(g, rho, t); d.e_b = (uint32_t)lat_index(g, rho, t + 1);
+          d.key0 = (uint32_t)sseed; d.key1 = (uint32_t)(sseed >> 32);
+          d.g = (uint32_t)g; d.rt = ((uint32_t)rho << 16) | (uint32_t)t;
+          d.job = 0; d.cnt_idx = (uint32_t)t; d.n_words = (uint32_t)N;
+          d.parity = (uint32_t)(t & 1);
+          pd.push_back(d);
+        }
+    if (npairs)
+      HIPCHK(hipMemcpy(d_pairs, pd.data(), sizeof(PairDesc) * npairs, hipMemcpyHostToDevice));
+
+    std::vector<MeasDesc> md((size_t)G * T);
+    for (int g = 0; g < G; ++g)
+      for (int t = 0; t < T; ++t) {
+        MeasDesc &d = md[(size_t)g * T + t];
+        d.spin_off = lat_index(g, 0, t) * (uint64_t)N;
+        d.rho_stride = (uint64_t)T * N;
+        d.j_off = (uint64_t)g * 3 * N;
+        d.out_off = ((uint64_t)g * T + t) * 64 * nfield;
+      }
+    HIPCHK(hipMemcpy(d_meas, md.data(), sizeof(MeasDesc) * md.size(), hipMemcpyHostToDevice));
+
+    h_rec.resize((size_t)G * T * 64 * nfield);
+    acc.assign((size_t)S * T, SampleAcc());
+    swap_att.assign((size_t)std::max(T - 1, 1), 0);
+    cosw.resize(L); sinw.resize(L);
+    for (int c = 0; c < L; ++c) {
+      cosw[c] = std::cos(2.0 * M_PI * c / L);
+      sinw[c] = std::sin(2.0 * M_PI * c / L);
+    }
+    reset_counters();
+  }
+
+  void build_thresholds(double b, ThrRec &r) const {
+#pragma clang fp contract(off)
+    std::memset(&r, 0, sizeof(r));
+    for (int sig = 0; sig < 2; ++sig) {
+      const double sv = sig ? 1.0 : -1.0;
+      uint32_t nfin = 0;
+      for (int u = 0; u < 7; ++u) {
+        const double dE = (double)(12 - 4 * u) + 2.0 * H * sv;
+        if (dE <= 0.0) break;
+        const double p = det_exp(-b * dE);
+        const uint64_t thr = (uint64_t)(p * 4294967296.0);
+        if (thr >= (1ull << 32)) throw Fail{SG_ERR_INVALID, "threshold overflow"};
+        r.t[sig][u] = (uint32_t)thr;
+        nfin = (uint32_t)(u + 1);
+      }
+      r.nfin[sig] = nfin;
+    }
+    if (!field && (r.nfin[0] != 3 || r.nfin[1] != 3))
+      throw Fail{SG_ERR_INVALID, "unexpected threshold table at H = 0"};
+  }
+
+  void reset_counters() {
+    JobState js;
+    std::memset(&js, 0, sizeof(js));
+    js.sweep = (uint32_t)sweep;
+    js.pt_every = (uint32_t)(T > 1 ? pt_every : 0);
+    HIPCHK(hipMemcpy(d_jobs, &js, sizeof(js), hipMemcpyHostToDevice));
+  }
+
+  // ------------------------------------------------------------ lattice
+  void init_lattice() {
+    for (int g = 0; g < G; ++g)
+      HIPCHK(launch_init_couplings(L, d_J, (uint64_t)g * 3 * N, (uint32_t)g, (uint32_t)cseed,
+                                   (uint32_t)(cseed >> 32), st));
+    HIPCHK(launch_init_spins(L, d_spins, d_lat, (int)n_lat(), st));
+    HIPCHK(hipStreamSynchronize(st));
+    sweep = 0;
+    n_meas = 0;
+    acc.assign(acc.size(), SampleAcc());
+    HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * std::max(T - 1, 1), st));
+    std::fill(swap_att.begin(), swap_att.end(), 0);
+    reset_counters();
+    energy_valid = false;
+    initialized = true;
+  }
+
+  // natural <-> checkerboard-split order
+  int cb(int s) const {
+    const int x = s % L, y = (s / L) % L, z = s / (L * L);
+    return ((x + y + z) & 1) * (N / 2) + (x >> 1) + (L / 2) * (y + L * z);
+  }
+
+  void get_spins(uint64_t *out) {
+    std::vector<uint64_t> raw(n_lat() * N);
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpy(raw.data(), d_spins, sizeof(uint64_t) * raw.size(), hipMemcpyDeviceToHost));
+    std::vector<int> perm(N);
+    for (int s = 0; s < N; ++s) perm[s] = cb(s);
+    for (size_t l = 0; l < n_lat(); ++l)
+      for (int s = 0; s < N; ++s) out[l * N + s] = raw[l * N + perm[s]];
+  }
+
+  void set_spins(const uint64_t *in) {
+    std::vector<uint64_t> raw(n_lat() * N);
+    std::vector<int> perm(N);
+    for (int s = 0; s < N; ++s) perm[s] = cb(s);
+    for (size_t l = 0; l < n_lat(); ++l)
+      for (int s = 0; s < N; ++s) raw[l * N + perm[s]] = in[l * N + s];
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpy(d_spins, raw.data(), sizeof(uint64_t) * raw.size(), hipMemcpyHostToDevice));
+    energy_valid = false;
+  }
+
+  // ------------------------------------------------------------ run loop
+  void launch_interval(int seg, bool want_energy) {
+    if (form == SG_FORM_PLAIN) {
+      // one group per launch (the per-sample-group drop-in form)
+      const int per_g = R * T;
+      const double fl = (double)seg * N * 64.0 * per_g;
+      for (int g = 0; g < G; ++g)
+        timed(0, fl, [&] {
+          HIPCHK(launch_sweep(L, field, d_spins, d_J, d_lat + (size_t)g * per_g, per_g, d_thr,
+                              d_jobs, seg, want_energy ? 1 : 0, d_energy, st));
+        });
+    } else {
+      const double fl = (double)seg * N * 64.0 * n_lat();
+      timed(0, fl, [&] {
+        HIPCHK(launch_sweep(L, field, d_spins, d_J, d_lat, (int)n_lat(), d_thr, d_jobs, seg,
+                            want_energy ? 1 : 0, d_energy, st));
+      });
+    }
+    HIPCHK(launch_advance(d_jobs, 1, seg, st));
+  }
+
+  void launch_pt_step() {
+    if (form == SG_FORM_PLAIN) {
+      const int per_g = R * (T - 1);
+      for (int g = 0; g < G; ++g)
+        timed(1, 0.0, [&] {
+          HIPCHK(launch_pt(d_spins, d_energy, d_pairs + (size_t)g * per_g, per_g, d_jobs,
+                           d_cnt, st));
+        });
+    } else {
+      timed(1, 0.0, [&] {
+        HIPCHK(launch_pt(d_spins, d_energy, d_pairs, npairs, d_jobs, d_cnt, st));
+      });
+    }
+  }
+
+  // Batched form: the PT step plus a full interval of pt_every sweeps is one
+  // hipGraph, replayed while the device sweep counter advances.
+  void run_graph_interval(int seg, bool want_energy) {
+    if (!graph || graph_seg != seg || graph_energy != want_energy) {
+      if (graph) HIPCHK(hipGraphExecDestroy(graph));
+      graph = nullptr;
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      launch_pt_step();
+      launch_interval(seg, want_energy);
+      HIPCHK(hipStreamEndCapture(st, &g));
+      hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      HIPCHK(e);
+      graph_seg = seg;
+      graph_energy = want_energy;
+    }
+    HIPCHK(hipGraphLaunch(graph, st));
+  }
+
+  void run(int64_t n_sweeps, sg_obs_sink sink, void *user) {
+    if (!initialized) throw Fail{SG_ERR_STATE, "lattice not initialised"};
+    if (n_sweeps < 0) throw Fail{SG_ERR_INVALID, "negative sweep count"};
+    if (sweep + n_sweeps > 0xFFFFFFFFll) throw Fail{SG_ERR_INVALID, "sweep counter overflow"};
+    const bool pt = pt_every > 0 && T > 1;
+    HIPCHK(hipEventRecord(ev0, st));
+    int64_t remaining = n_sweeps;
+    while (remaining > 0) {
+      const int64_t n = sweep;
+      int64_t seg = remaining;
+      if (pt) seg = std::min<int64_t>(seg, pt_every - n % pt_every);
+      if (meas_every > 0) seg = std::min<int64_t>(seg, meas_every - n % meas_every);
+      const int64_t next = n + seg;
+      const bool want_energy = pt && next % pt_every == 0;
+      const bool pt_now = pt && n % pt_every == 0;
+      if (pt_now && !energy_valid)
+        HIPCHK(launch_energy(L, d_spins, d_J, d_lat, (int)n_lat(), d_energy, st));
+      if (form == SG_FORM_BATCHED && pt_now && seg == pt_every && !profiling) {
+        run_graph_interval((int)seg, want_energy);
+      } else {
+        if (pt_now) launch_pt_step();
+        launch_interval((int)seg, want_energy);
+      }
+      if (pt_now) {
+        const int parity = (int)((n / pt_every) & 1);
+        for (int t = parity; t + 1 < T; t += 2) swap_att[t] += (uint64_t)G * R * 64;
+      }
+      sweep = next;
+      energy_valid = want_energy;
+      flips += (double)seg * N * 64.0 * G * R * T;
+      remaining -= seg;
+      if (meas_every > 0 && next % meas_every == 0) measure(sink, user);
+    }
+    HIPCHK(hipEventRecord(ev1, st));
+    HIPCHK(hipEventSynchronize(ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+    kernel_ms += ms;
+    collect_profile();
+  }
+
+  void measure(sg_obs_sink sink, void *user) {
+    timed(2, 0.0, [&] {
+      HIPCHK(launch_measure(L, R, d_spins, d_J, d_meas, G * T, d_rec, nfield, st));
+    });
+    HIPCHK(hipMemcpyAsync(h_rec.data(), d_rec, sizeof(int32_t) * h_rec.size(),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    accumulate(h_rec.data());
+    ++n_meas;
+    if (sink) sink(user, sweep, h_rec.data(), (int64_t)G * T * 64, nfield);
+  }
+
+  void accumulate(const int32_t *rec) {
+    for (int g = 0; g < G; ++g)
+      for (int t = 0; t < T; ++t)
+        for (int l = 0; l < 64; ++l) {
+          const int sample = g * 64 + l;
+          if (sample >= S) continue;
+          const int32_t *f = rec + (((size_t)g * T + t) * 64 + l) * nfield;
+          SampleAcc &a = acc[(size_t)sample * T + t];
+          double se = 0, sm = 0;
+          for (int rho = 0; rho < R; ++rho) {
+            const double M = (double)(2 * f[R + rho] - N);
+            se += ((double)(2 * f[rho] - 3 * N) - H * M) / N;
+            sm += M / N;
+          }
+          a.sum_e += se / R;
+          a.sum_m += sm / R;
+          if (R >= 2) {
+            const int64_t Q = (int64_t)N - 2 * (int64_t)f[2 * R];
+            a.sum_Q += Q;
+            const unsigned __int128 Q2 = (unsigned __int128)(Q * Q);
+            a.sum_Q2 += Q2;
+            a.sum_Q4 += Q2 * Q2;
+            a.sum_absq += (double)(Q < 0 ? -Q : Q) / N;
+            double q1 = 0;
+            for (int dir = 0; dir < 3; ++dir) {
+              const int32_t *W = f + 2 * R + 1 + dir * L;
+              double re = 0, im = 0;
+              for (int c = 0; c < L; ++c) {
+                re += cosw[c] * W[c];
+                im += sinw[c] * W[c];
+              }
+              re *= -2.0 / N;
+              im *= -2.0 / N;
+              q1 += re * re + im * im;
+            }
+            a.sum_q1sq += q1 / 3.0;
+          }
+          a.n += 1;
+        }
+  }
+
+  void results(double *res) const {
+    const double Nd = N;
+    for (int t = 0; t < T; ++t) {
+      double e = 0, m = 0, aq = 0, q2 = 0, q4 = 0, c1 = 0;
+      int64_t nm = 0;
+      for (int s = 0; s < S; ++s) {
+        const SampleAcc &a = acc[(size_t)s * T + t];
+        if (a.n == 0) continue;
+        const double n = (double)a.n;
+        e += a.sum_e / n;
+        m += a.sum_m / n;
+        aq += a.sum_absq / n;
+        q2 += (double)a.sum_Q2 / n / (Nd * Nd);
+        q4 += (double)a.sum_Q4 / n / (Nd * Nd * Nd * Nd);
+        c1 += a.sum_q1sq / n;
+        nm = a.n;
+      }
+      double *r = res + (size_t)t * SG_NRES;
+      const double ns = S;
+      e /= ns; m /= ns; aq /= ns; q2 /= ns; q4 /= ns; c1 /= ns;
+      r[SG_RES_BETA] = beta[t];
+      r[SG_RES_E] = e;
+      r[SG_RES_M] = m;
+      r[SG_RES_Q] = aq;
+      r[SG_RES_Q2] = q2;
+      r[SG_RES_Q4] = q4;
+      r[SG_RES_BINDER] = q2 > 0 ? 0.5 * (3.0 - q4 / (q2 * q2)) : NAN;
+      r[SG_RES_CHI0] = q2;
+      r[SG_RES_CHI1] = c1;
+      r[SG_RES_XI_L] = (c1 > 0 && q2 / c1 > 1.0)
+                           ? std::sqrt(q2 / c1 - 1.0) / (2.0 * std::sin(M_PI / L)) / L
+                           : NAN;
+      r[SG_RES_NMEAS] = (double)nm;
+    }
+  }
+
+  // ------------------------------------------------------------ checkpoint
+  struct CkHeader {
+    char magic[8];
+    uint32_t version, L, G, R, T, S, pt_every, meas_every;
+    double H;
+    uint64_t cseed, sseed;
+    int64_t sweep, n_meas;
+    uint64_t payload_bytes;
+  };
+
+  size_t ck_size() const {
+    return sizeof(CkHeader) + sizeof(double) * T + sizeof(uint64_t) * n_lat() * N +
+           sizeof(uint64_t) * (size_t)G * 3 * N + sizeof(uint64_t) * 2 * std::max(T - 1, 1) +
+           sizeof(SampleAcc) * acc.size() + sizeof(uint64_t);
+  }
+
+  void save(void *buf, size_t *size) {
+    const size_t need = ck_size();
+    if (!buf) { *size = need; return; }
+    if (*size < need) throw Fail{SG_ERR_INVALID, "checkpoint buffer too small"};
+    if (!initialized) throw Fail{SG_ERR_STATE, "lattice not initialised"};
+    char *p = (char *)buf;
+    CkHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, "SGCHKPT1", 8);
+    h.version = 1; h.L = L; h.G = G; h.R = R; h.T = T; h.S = S;
+    h.pt_every = pt_every; h.meas_every = meas_every; h.H = H; h.cseed = cseed; h.sseed = sseed;
+    h.sweep = sweep; h.n_meas = n_meas;
+    h.payload_bytes = need - sizeof(CkHeader) - sizeof(uint64_t);
+    std::memcpy(p, &h, sizeof(h)); p += sizeof(h);
+    std::memcpy(p, beta.data(), sizeof(double) * T); p += sizeof(double) * T;
+    get_spins((uint64_t *)p); p += sizeof(uint64_t) * n_lat() * N;
+    HIPCHK(hipMemcpy(p, d_J, sizeof(uint64_t) * (size_t)G * 3 * N, hipMemcpyDeviceToHost));
+    p += sizeof(uint64_t) * (size_t)G * 3 * N;
+    std::vector<uint64_t> cnt(std::max(T - 1, 1));
+    HIPCHK(hipMemcpy(cnt.data(), d_cnt, sizeof(uint64_t) * cnt.size(), hipMemcpyDeviceToHost));
+    std::memcpy(p, cnt.data(), sizeof(uint64_t) * cnt.size()); p += sizeof(uint64_t) * cnt.size();
+    std::memcpy(p, swap_att.data(), sizeof(uint64_t) * cnt.size()); p += sizeof(uint64_t) * cnt.size();
+    std::memcpy(p, acc.data(), sizeof(SampleAcc) * acc.size()); p += sizeof(SampleAcc) * acc.size();
+    const uint64_t ck = fnv1a(buf, need - sizeof(uint64_t));
+    std::memcpy(p, &ck, sizeof(ck));
+    *size = need;
+  }
+
+  void load(const void *buf, size_t size) {
+    const size_t need = ck_size();
+    if (size != need) throw Fail{SG_ERR_FORMAT, "checkpoint size mismatch"};
+    const char *p = (const char *)buf;
+    CkHeader h;
+    std::memcpy(&h, p, sizeof(h));
+    if (std::memcmp(h.magic, "SGCHKPT1", 8) != 0 || h.version != 1)
+      throw Fail{SG_ERR_FORMAT, "checkpoint magic/version mismatch"};
+    uint64_t ck;
+    std::memcpy(&ck, p + need - sizeof(uint64_t), sizeof(ck));
+    if (ck != fnv1a(buf, need - sizeof(uint64_t)))
+      throw Fail{SG_ERR_FORMAT, "checkpoint checksum mismatch"};
+    if ((int)h.L != L || (int)h.G != G || (int)h.R != R || (int)h.T != T || (int)h.S != S ||
+        (int)h.pt_every != pt_every || (int)h.meas_every != meas_every || h.H != H ||
+        h.cseed != cseed || h.sseed != sseed)
+      throw Fail{SG_ERR_FORMAT, "checkpoint parameters differ from this handle"};
+    p += sizeof(h);
+    if (std::memcmp(p, beta.data(), sizeof(double) * T) != 0)
+      throw Fail{SG_ERR_FORMAT, "checkpoint ladder differs"};
+    p += sizeof(double) * T;
+    initialized = true;
+    set_spins((const uint64_t *)p); p += sizeof(uint64_t) * n_lat() * N;
+    HIPCHK(hipMemcpy(d_J, p, sizeof(uint64_t) * (size_t)G * 3 * N, hipMemcpyHostToDevice));
+    p += sizeof(uint64_t) * (size_t)G * 3 * N;
+    const size_t nc = std::max(T - 1, 1);
+    HIPCHK(hipMemcpy(d_cnt, p, sizeof(uint64_t) * nc, hipMemcpyHostToDevice)); p += sizeof(uint64_t) * nc;
+    std::memcpy(swap_att.data(), p, sizeof(uint64_t) * nc); p += sizeof(uint64_t) * nc;
+    std::memcpy(acc.data(), p, sizeof(SampleAcc) * acc.size());
+    sweep = h.sweep;
+    n_meas = h.n_meas;
+    reset_counters();
+    energy_valid = false;
+  }
+};
+
+// ---------------------------------------------------------------- C-ABI
+template <class F>
+static sg_status guarded(sg_handle *h, F &&f) {
+  if (!h) return SG_ERR_INVALID;
+  try {
+    f();
+    h->err.clear();
+    return SG_OK;
+  } catch (const Fail &e) {
+    h->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    h->err = "host allocation failed";
+    return SG_ERR_NOMEM;
+  }
+}
+
+extern "C" {
+
+void sg_params_default(sg_params *p) {
+  static double ref_betas[24];
+  for (int i = 0; i < 24; ++i) ref_betas[i] = 2.0 * std::pow(2.0, -4.0 * i / 23.0);
+  std::memset(p, 0, sizeof(*p));
+  p->L = 16; p->n_samples = 2048; p->n_temps = 24; p->n_replicas = 2;
+  p->betas = ref_betas; p->H = 0.0;
+  p->coupling_seed = 0x5EED0001ull; p->spin_seed = 0x5EED0002ull;
+  p->pt_every = 16; p->meas_every = 1024; p->device = 0; p->form = SG_FORM_PLAIN;
+}
+
+sg_status sg_create(const sg_params *p, sg_handle **out) {
+  if (!out) return SG_ERR_INVALID;
+  *out = nullptr;
+  g_create_error.clear();
+  auto bad = [](const char *m) { g_create_error = m; return SG_ERR_INVALID; };
+  if (!p) return bad("params is NULL");
+  if (p->L < 2 || p->L > kMaxL || (p->L & 1)) return bad("L must be even and in [2, 48]");
+  if (!lds_kernel_supported(p->L)) return bad("this build sweeps L <= 16 only");
+  if (p->n_samples < 1) return bad("n_samples must be >= 1");
+  if (p->n_temps < 1 || p->n_temps > 4096) return bad("n_temps must be in [1, 4096]");
+  if (p->n_replicas < 1 || p->n_replicas > 8) return bad("n_replicas must be in [1, 8]");
+  if (!(p->H >= 0.0 && p->H <= 4.0)) return bad("H out of range [0, 4]");
+  if (p->pt_every < 0 || p->meas_every < 0) return bad("negative cadence");
+  if (!p->betas && !(p->T_min > 0.0 && p->T_max > 0.0)) return bad("need betas or T_min/T_max > 0");
+  if (p->form != SG_FORM_PLAIN && p->form != SG_FORM_BATCHED) return bad("unknown form");
+  sg_handle *h = new (std::nothrow) sg_handle();
+  if (!h) return bad("host allocation failed");
+  try {
+    h->create(*p);
+  } catch (const Fail &e) {
+    g_create_error = e.msg;
+    delete h;
+    return e.code;
+  }
+  *out = h;
+  return SG_OK;
+}
+
+sg_status sg_destroy(sg_handle *h) {
+  if (!h) return SG_ERR_INVALID;
+  delete h;
+  return SG_OK;
+}
+
+const char *sg_last_error(const sg_handle *h) {
+  return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+sg_status sg_init_lattice(sg_handle *h) {
+  return guarded(h, [&] { h->init_lattice(); });
+}
+
+sg_status sg_run(sg_handle *h, int64_t n_sweeps, sg_obs_sink sink, void *user) {
+  return guarded(h, [&] { h->run(n_sweeps, sink, user); });
+}
+
+sg_status sg_save_checkpoint(sg_handle *h, void *buf, size_t *size) {
+  if (!size) return SG_ERR_INVALID;
+  return guarded(h, [&] { h->save(buf, size); });
+}
+
+sg_status sg_load_checkpoint(sg_handle *h, const void *buf, size_t size) {
+  if (!buf) return SG_ERR_INVALID;
+  return guarded(h, [&] { h->load(buf, size); });
+}
+
+int32_t sg_nfield(const sg_handle *h) { return h ? h->nfield : 0; }
+int32_t sg_n_groups(const sg_handle *h) { return h ? h->G : 0; }
+int64_t sg_sweep_count(const sg_handle *h) { return h ? h->sweep : -1; }
+
+sg_status sg_get_betas(const sg_handle *h, double *betas) {
+  if (!h || !betas) return SG_ERR_INVALID;
+  std::memcpy(betas, h->beta.data(), sizeof(double) * h->T);
+  return SG_OK;
+}
+
+sg_status sg_get_spins(sg_handle *h, uint64_t *out) {
+  if (!out) return SG_ERR_INVALID;
+  return guarded(h, [&] { h->get_spins(out); });
+}
+
+sg_status sg_set_spins(sg_handle *h, const uint64_t *in) {
+  if (!in) return SG_ERR_INVALID;
+  return guarded(h, [&] {
+    h->set_spins(in);
+    h->initialized = true;
+  });
+}
+
+sg_status sg_get_couplings(sg_handle *h, uint64_t *out) {
+  if (!out) return SG_ERR_INVALID;
+  return guarded(h, [&] {
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipMemcpy(out, h->d_J, sizeof(uint64_t) * (size_t)h->G * 3 * h->N,
+                     hipMemcpyDeviceToHost));
+  });
+}
+
+sg_status sg_set_couplings(sg_handle *h, const uint64_t *in) {
+  if (!in) return SG_ERR_INVALID;
+  return guarded(h, [&] {
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipMemcpy(h->d_J, in, sizeof(uint64_t) * (size_t)h->G * 3 * h->N,
+                     hipMemcpyHostToDevice));
+    h->energy_valid = false;
+  });
+}
+
+sg_status sg_get_energies(sg_handle *h, int32_t *U, int32_t *K) {
+  if (!U || !K) return SG_ERR_INVALID;
+  return guarded(h, [&] {
+    if (!h->energy_valid)
+      HIPCHK(launch_energy(h->L, h->d_spins, h->d_J, h->d_lat, (int)h->n_lat(), h->d_energy, h->st));
+    h->energy_valid = true;
+    std::vector<int32_t> e(h->n_lat() * 128);
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipMemcpy(e.data(), h->d_energy, sizeof(int32_t) * e.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h->n_lat(); ++i) {
+      std::memcpy(U + i * 64, e.data() + i * 128, sizeof(int32_t) * 64);
+      std::memcpy(K + i * 64, e.data() + i * 128 + 64, sizeof(int32_t) * 64);
+    }
+  });
+}
+
+sg_status sg_get_swap_counts(sg_handle *h, uint64_t *accepted, uint64_t *attempted) {
+  if (!accepted || !attempted) return SG_ERR_INVALID;
+  return guarded(h, [&] {
+    if (h->T < 2) return;
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipMemcpy(accepted, h->d_cnt, sizeof(uint64_t) * (h->T - 1), hipMemcpyDeviceToHost));
+    std::memcpy(attempted, h->swap_att.data(), sizeof(uint64_t) * (h->T - 1));
+  });
+}
+
+sg_status sg_get_results(sg_handle *h, double *res) {
+  if (!res) return SG_ERR_INVALID;
+  return guarded(h, [&] { h->results(res); });
+}
+
+sg_status sg_get_timing(sg_handle *h, double *kernel_ms, double *flips) {
+  if (!kernel_ms || !flips) return SG_ERR_INVALID;
+  return guarded(h, [&] {
+    *kernel_ms = h->kernel_ms;
+    *flips = h->flips;
+  });
+}
+
+sg_status sg_set_profiling(sg_handle *h, int32_t on) {
+  return guarded(h, [&] { h->profiling = on != 0; });
+}
+
+sg_status sg_get_kernel_stats(sg_handle *h, int32_t kind, int64_t *launches, double *total_ms,
+                              double *flips) {
+  if (!launches || !total_ms || !flips || kind < 0 || kind > 2) return SG_ERR_INVALID;
+  return guarded(h, [&] {
+    h->collect_profile();
+    *launches = h->kstat[kind].launches;
+    *total_ms = h->kstat[kind].ms;
+    *flips = h->kstat[kind].flips;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,312 @@
+"""Python binding of libsg (ctypes over include/sg.h) and a mirror of the
+reference's host object.
+
+`Sim` wraps one sg_handle.  `eamsim` keeps the reference's method names and
+argument meaning (Program/src/eamsim.cuh:66-110): eamsim(H), init_lattice(),
+load_checkpoint(file), run_simulation(file), save_checkpoint(file).  Errors
+raise SGError (the reference throws `error`, Program/src/util.cuh:46-54).
+
+There is no fallback: if libsg.so is missing or cannot load, every entry
+point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsg.so")
+
+SG_FORM_PLAIN = 0
+SG_FORM_BATCHED = 1
+RES_COLUMNS = ["beta", "e", "m", "q", "q2", "q4", "binder", "chi0", "chi1", "xi_L", "n_meas"]
+NRES = len(RES_COLUMNS)
+
+# the reference's temperature ladder, eamsim.cu:728-737
+REF_BETAS = 2.0 * 2.0 ** (-4.0 * np.arange(24) / 23.0)
+
+
+class SGError(RuntimeError):
+    pass
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [
+        ("L", ctypes.c_int32), ("n_samples", ctypes.c_int32), ("n_temps", ctypes.c_int32),
+        ("n_replicas", ctypes.c_int32), ("betas", ctypes.POINTER(ctypes.c_double)),
+        ("T_min", ctypes.c_double), ("T_max", ctypes.c_double), ("H", ctypes.c_double),
+        ("coupling_seed", ctypes.c_uint64), ("spin_seed", ctypes.c_uint64),
+        ("pt_every", ctypes.c_int32), ("meas_every", ctypes.c_int32),
+        ("device", ctypes.c_int32), ("form", ctypes.c_int32),
+    ]
+
+
+SINK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
+                        ctypes.c_int64, ctypes.c_int32)
+
+# every symbol include/sg.h declares
+EXPORTS = [
+    "sg_params_default", "sg_create", "sg_destroy", "sg_last_error", "sg_init_lattice",
+    "sg_run", "sg_save_checkpoint", "sg_load_checkpoint", "sg_nfield", "sg_n_groups",
+    "sg_sweep_count", "sg_get_betas", "sg_get_spins", "sg_set_spins", "sg_get_couplings",
+    "sg_set_couplings", "sg_get_energies", "sg_get_swap_counts", "sg_get_results",
+    "sg_get_timing", "sg_set_profiling", "sg_get_kernel_stats",
+]
+
+_lib = None
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libsg.so and declare its signatures (no device call is made)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SGError(f"{path} is missing: run `python spin-glass_amd/build_sg.py`")
+    L = ctypes.CDLL(path)
+    L.sg_params_default.argtypes = [ctypes.POINTER(_Params)]
+    L.sg_create.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(_vp)]
+    L.sg_destroy.argtypes = [_vp]
+    L.sg_last_error.argtypes = [_vp]
+    L.sg_last_error.restype = ctypes.c_char_p
+    L.sg_init_lattice.argtypes = [_vp]
+    L.sg_run.argtypes = [_vp, ctypes.c_int64, SINK, _vp]
+    L.sg_save_checkpoint.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_size_t)]
+    L.sg_load_checkpoint.argtypes = [_vp, _vp, ctypes.c_size_t]
+    L.sg_nfield.argtypes = [_vp]
+    L.sg_n_groups.argtypes = [_vp]
+    L.sg_sweep_count.argtypes = [_vp]
+    L.sg_sweep_count.restype = ctypes.c_int64
+    L.sg_get_betas.argtypes = [_vp, _dp]
+    for n in ("sg_get_spins", "sg_set_spins", "sg_get_couplings", "sg_set_couplings"):
+        getattr(L, n).argtypes = [_vp, _u64p]
+    L.sg_get_energies.argtypes = [_vp, _i32p, _i32p]
+    L.sg_get_swap_counts.argtypes = [_vp, _u64p, _u64p]
+    L.sg_get_results.argtypes = [_vp, _dp]
+    L.sg_get_timing.argtypes = [_vp, _dp, _dp]
+    L.sg_set_profiling.argtypes = [_vp, ctypes.c_int32]
+    L.sg_get_kernel_stats.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), _dp, _dp]
+    _lib = L
+    return L
+
+
+def _check(status, h=None):
+    if status != 0:
+        msg = load_library().sg_last_error(h)
+        raise SGError(f"libsg status {status}: {msg.decode() if msg else ''}")
+
+
+class Sim:
+    """One libsg handle: L, samples (64 per group), temperatures, replicas."""
+
+    def __init__(self, L=16, n_samples=64, n_replicas=2, betas=None, T_range=None, H=0.0,
+                 coupling_seed=0x5EED0001, spin_seed=0x5EED0002, pt_every=16,
+                 meas_every=0, device=0, form=SG_FORM_PLAIN):
+        lib = load_library()
+        p = _Params()
+        p.L, p.n_samples, p.n_replicas = L, n_samples, n_replicas
+        if betas is not None:
+            self._betas = np.ascontiguousarray(betas, dtype=np.float64)
+            p.n_temps = len(self._betas)
+            p.betas = self._betas.ctypes.data_as(_dp)
+        else:
+            tmin, tmax, nt = T_range
+            self._betas = None
+            p.n_temps = nt
+            p.T_min, p.T_max = tmin, tmax
+        p.H = H
+        p.coupling_seed, p.spin_seed = coupling_seed, spin_seed
+        p.pt_every, p.meas_every, p.device, p.form = pt_every, meas_every, device, form
+        h = _vp()
+        _check(lib.sg_create(ctypes.byref(p), ctypes.byref(h)), None)
+        self._h = h
+        self.L, self.N, self.R, self.T = L, L ** 3, n_replicas, p.n_temps
+        self.n_samples = n_samples
+        self.G = lib.sg_n_groups(h)
+        self.nfield = lib.sg_nfield(h)
+        self.H = H
+        self.records = []
+        self.keep_records = True
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().sg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _c(self, st):
+        _check(st, self._h)
+
+    @property
+    def betas(self):
+        b = np.zeros(self.T, dtype=np.float64)
+        self._c(load_library().sg_get_betas(self._h, b.ctypes.data_as(_dp)))
+        return b
+
+    @property
+    def sweep(self):
+        return load_library().sg_sweep_count(self._h)
+
+    def init_lattice(self):
+        self._c(load_library().sg_init_lattice(self._h))
+
+    def run(self, n_sweeps, sink=None):
+        """Run n_sweeps; each measurement is appended to self.records as an
+        int32 array [group][temp][lane][field] and passed to sink(sweep, rec)."""
+        def _cb(user, sweep, rec, n, nfield):
+            arr = np.ctypeslib.as_array(rec, shape=(n * nfield,)).copy()
+            arr = arr.reshape(self.G, self.T, 64, nfield)
+            if self.keep_records:
+                self.records.append((sweep, arr))
+            if sink is not None:
+                sink(sweep, arr)
+        cb = SINK(_cb)
+        self._c(load_library().sg_run(self._h, int(n_sweeps), cb, None))
+
+    def spins(self):
+        out = np.zeros((self.G, self.R, self.T, self.N), dtype=np.uint64)
+        self._c(load_library().sg_get_spins(self._h, out.ctypes.data_as(_u64p)))
+        return out
+
+    def set_spins(self, S):
+        S = np.ascontiguousarray(S, dtype=np.uint64)
+        if S.size != self.G * self.R * self.T * self.N:
+            raise SGError("spin array has the wrong size")
+        self._c(load_library().sg_set_spins(self._h, S.ctypes.data_as(_u64p)))
+
+    def couplings(self):
+        out = np.zeros((self.G, 3, self.N), dtype=np.uint64)
+        self._c(load_library().sg_get_couplings(self._h, out.ctypes.data_as(_u64p)))
+        return out
+
+    def set_couplings(self, J):
+        J = np.ascontiguousarray(J, dtype=np.uint64)
+        if J.size != self.G * 3 * self.N:
+            raise SGError("coupling array has the wrong size")
+        self._c(load_library().sg_set_couplings(self._h, J.ctypes.data_as(_u64p)))
+
+    def energies(self):
+        U = np.zeros((self.G, self.R, self.T, 64), dtype=np.int32)
+        K = np.zeros_like(U)
+        self._c(load_library().sg_get_energies(self._h, U.ctypes.data_as(_i32p),
+                                               K.ctypes.data_as(_i32p)))
+        return U, K
+
+    def swap_counts(self):
+        n = max(self.T - 1, 1)
+        a = np.zeros(n, dtype=np.uint64)
+        b = np.zeros(n, dtype=np.uint64)
+        self._c(load_library().sg_get_swap_counts(self._h, a.ctypes.data_as(_u64p),
+                                                  b.ctypes.data_as(_u64p)))
+        return a[: self.T - 1], b[: self.T - 1]
+
+    def results(self):
+        r = np.zeros((self.T, NRES), dtype=np.float64)
+        self._c(load_library().sg_get_results(self._h, r.ctypes.data_as(_dp)))
+        return r
+
+    def timing(self):
+        ms = ctypes.c_double()
+        fl = ctypes.c_double()
+        self._c(load_library().sg_get_timing(self._h, ctypes.byref(ms), ctypes.byref(fl)))
+        return ms.value, fl.value
+
+    def set_profiling(self, on=True):
+        self._c(load_library().sg_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_stats(self, kind=0):
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        fl = ctypes.c_double()
+        self._c(load_library().sg_get_kernel_stats(self._h, kind, ctypes.byref(n),
+                                                   ctypes.byref(ms), ctypes.byref(fl)))
+        return n.value, ms.value, fl.value
+
+    def save_checkpoint_bytes(self) -> bytes:
+        lib = load_library()
+        n = ctypes.c_size_t(0)
+        self._c(lib.sg_save_checkpoint(self._h, None, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._c(lib.sg_save_checkpoint(self._h, buf, ctypes.byref(n)))
+        return buf.raw[: n.value]
+
+    def load_checkpoint_bytes(self, data: bytes):
+        buf = ctypes.create_string_buffer(data, len(data))
+        self._c(load_library().sg_load_checkpoint(self._h, buf, len(data)))
+
+
+# ------------------------------------------------------------------ drop-in
+OBS_DTYPE = np.dtype([("e", "f4", (2,)), ("m", "f4", (2,)), ("q_0", "f4"),
+                      ("q_1_r", "f4", (3,)), ("q_1_i", "f4", (3,))])
+
+
+def obs_records(rec: np.ndarray, L: int, R: int, H: float, n_samples: int) -> np.ndarray:
+    """Integer measurement records [group][temp][lane][field] -> the
+    reference's float32 obs_s array [sample][temp] (Program/src/eamsim.cuh:
+    55-62, eamsim.cu:408-415 and 504-509)."""
+    G, T = rec.shape[0], rec.shape[1]
+    N = L ** 3
+    r = rec.transpose(0, 2, 1, 3).reshape(G * 64, T, -1)[:n_samples].astype(np.float64)
+    out = np.zeros((n_samples, T), dtype=OBS_DTYPE)
+    U, K = r[..., :R], r[..., R:2 * R]
+    M = 2 * K - N
+    E = (2 * U - 3 * N) - H * M
+    ncp = min(R, 2)
+    out["e"][..., :ncp] = (E / N)[..., :ncp]
+    out["m"][..., :ncp] = (M / N)[..., :ncp]
+    if R >= 2:
+        W = r[..., 2 * R]
+        out["q_0"] = (N - 2 * W) / N
+        c = np.arange(L)
+        cw, sw = np.cos(2 * np.pi * c / L), np.sin(2 * np.pi * c / L)
+        for d in range(3):
+            Wd = r[..., 2 * R + 1 + d * L: 2 * R + 1 + (d + 1) * L]
+            out["q_1_r"][..., d] = -2.0 / N * (Wd @ cw)
+            out["q_1_i"][..., d] = -2.0 / N * (Wd @ sw)
+    return out
+
+
+class eamsim:
+    """Drop-in for the reference's `eamsim` (Program/src/eamsim.cuh:66-110).
+
+    eamsim(H) fixes the reference's configuration (L=16, 2048 samples, 2
+    copies, the 24-temperature ladder, PT every 16 sweeps, a measurement
+    every 1024, 262,144 sweeps per run_simulation call); keyword arguments
+    override any of them."""
+
+    def __init__(self, H: float, L=16, n_samples=2048, n_replicas=2, betas=REF_BETAS,
+                 pt_every=16, meas_every=1024, sweeps_per_file=262144,
+                 coupling_seed=0x5EED0001, spin_seed=0x5EED0002, form=SG_FORM_BATCHED):
+        if not (0.0 <= H <= 4.0):
+            raise SGError("H out of range")
+        self.H = float(H)
+        self.sweeps_per_file = sweeps_per_file
+        self.sim = Sim(L=L, n_samples=n_samples, n_replicas=n_replicas, betas=betas, H=H,
+                       coupling_seed=coupling_seed, spin_seed=spin_seed, pt_every=pt_every,
+                       meas_every=meas_every, form=form)
+        self.sim.keep_records = False
+
+    def init_lattice(self):
+        self.sim.init_lattice()
+
+    def save_checkpoint(self, f):
+        f.write(self.sim.save_checkpoint_bytes())
+
+    def load_checkpoint(self, f):
+        self.sim.load_checkpoint_bytes(f.read())
+
+    def run_simulation(self, f):
+        s = self.sim
+
+        def sink(sweep, rec):
+            f.write(obs_records(rec, s.L, s.R, s.H, s.n_samples).tobytes())
+
+        s.run(self.sweeps_per_file, sink=sink)
