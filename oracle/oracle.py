@@ -45,6 +45,7 @@ def lib():
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_init.argtypes = [ctypes.c_void_p]
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        L.orc_pt_step.argtypes = [ctypes.c_void_p, ctypes.c_int64]
         L.orc_sweep_count.argtypes = [ctypes.c_void_p]
         L.orc_sweep_count.restype = ctypes.c_int64
         L.orc_nfield.argtypes = [ctypes.c_void_p]
@@ -101,6 +102,10 @@ def lattice_energy(L, S, J):
     K = np.zeros(64, dtype=np.int32)
     lib().orc_lattice_energy(L, _ptr(S, _u64p), _ptr(J, _u64p), _ptr(U, _i32p), _ptr(K, _i32p))
     return U, K
+
+
+def pt_step(sim, b: int) -> None:
+    lib().orc_pt_step(sim._h, int(b))
 
 
 def set_threads(n: int) -> None:

@@ -309,7 +309,7 @@ void orc_init(orc_sim *s) {
 
 int orc_nfield(const orc_sim *s) { return 2 * s->R + 1 + 3 * s->L; }
 
-static void pt_step(orc_sim *s, int64_t b) {
+void orc_pt_step(orc_sim *s, int64_t b) {
   const int N = s->N, T = s->T;
   const int nl = s->G * s->R;
   int32_t *U = (int32_t *)malloc(sizeof(int32_t) * 64 * (size_t)T * nl);
@@ -402,7 +402,7 @@ void orc_run(orc_sim *s, int64_t n_sweeps) {
   const int nl = s->G * s->R * s->T;
   for (int64_t it = 0; it < n_sweeps; ++it) {
     int64_t n = s->sweep;
-    if (s->pt_every > 0 && s->T > 1 && n % s->pt_every == 0) pt_step(s, n / s->pt_every);
+    if (s->pt_every > 0 && s->T > 1 && n % s->pt_every == 0) orc_pt_step(s, n / s->pt_every);
 #pragma omp parallel for schedule(dynamic) num_threads(g_threads)
     for (int idx = 0; idx < nl; ++idx) {
       int t = idx % s->T, gr = idx / s->T;

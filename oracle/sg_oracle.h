@@ -89,6 +89,7 @@ void orc_destroy(orc_sim *sim);
 void orc_set_threads(int n);
 void orc_init(orc_sim *sim);
 void orc_run(orc_sim *sim, int64_t n_sweeps);
+void orc_pt_step(orc_sim *sim, int64_t b);   /* one PT step b on the current state */
 int64_t orc_sweep_count(const orc_sim *sim);
 int orc_nfield(const orc_sim *sim);
 int64_t orc_n_meas(const orc_sim *sim);

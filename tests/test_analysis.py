@@ -1,0 +1,102 @@
+"""The res.dat writer against hand-computed values (the reference's
+analyze-all.py formulas evaluated in plain Python on a tiny input), and the
+obs-record conversion against direct sums over spins."""
+import math
+import os
+
+import numpy as np
+
+import analysis
+import sg
+
+
+def _tiny(tmp_path):
+    # 4 measurements x 2 samples x 2 temperatures; the first 2 are discarded
+    d = np.zeros((4, 2, 2), dtype=analysis.OBS_DTYPE)
+    rng = np.random.default_rng(5)
+    for f, shape in (("e", (2,)), ("m", (2,)), ("q_1_r", (3,)), ("q_1_i", (3,))):
+        d[f] = rng.uniform(-0.5, 0.5, size=(4, 2, 2) + shape).astype(np.float32)
+    d["q_0"] = rng.uniform(-0.9, 0.9, size=(4, 2, 2)).astype(np.float32)
+    d["q_0"][:2] = 99.0       # must be cut away as thermalisation
+    sub = tmp_path / "08"
+    sub.mkdir()
+    (sub / "0.1250-obs.bin").write_bytes(d.tobytes())
+    return d[2:].astype(analysis.OBS_DTYPE)
+
+
+def test_res_dat_matches_hand_computation(tmp_path):
+    kept = _tiny(tmp_path)
+    out = analysis.analyze_obs(str(tmp_path), 8, 0.125, n_samples=2, n_temps=2,
+                               betas=[1.5, 0.5])
+    lines = open(out).read().splitlines()
+    assert lines[0] == "# tot_MC_steps : 4,096"
+    rows = np.array([[float(line[i:i + 16]) for i in range(0, 9 * 16, 16)] for line in lines[1:]])
+    L = 8
+    for t in range(2):
+        def col(field, comp=None):
+            v = kept[field][:, :, t] if comp is None else kept[field][:, :, t, comp]
+            return [[float(v[m, s]) for m in range(2)] for s in range(2)]
+
+        def mean(a):
+            return sum(a) / len(a)
+
+        def var(a):
+            mu = mean(a)
+            return sum((x - mu) ** 2 for x in a) / len(a)
+
+        chi0_s = [mean([q * q for q in col("q_0")[s]]) for s in range(2)]
+        chi0, chi0_v = mean(chi0_s), var(chi0_s)
+        re_s = [[mean([q * q for q in col("q_1_r", d)[s]]) for s in range(2)] for d in range(3)]
+        im_s = [[mean([q * q for q in col("q_1_i", d)[s]]) for s in range(2)] for d in range(3)]
+        chi1 = mean([mean(re_s[d]) + mean(im_s[d]) for d in range(3)])
+        chi1_v = mean([var(re_s[d]) + var(im_s[d]) for d in range(3)])
+        ratio = chi0 / chi1
+        ratio_v = ratio ** 2 * (chi0_v / chi0 ** 2 + chi1_v / chi1 ** 2)
+        xi = math.sqrt(ratio - 1) / (2 * math.sin(math.pi / L)) / L if ratio > 1 else float("nan")
+        xi_e = math.sqrt((xi / (2 * (ratio - 1))) ** 2 * ratio_v / 2) if ratio > 1 else float("nan")
+        e_s = [mean([float(kept["e"][m, s, t, c]) for m in range(2) for c in range(2)])
+               for s in range(2)]
+        m_s = [mean([float(kept["m"][m, s, t, c]) for m in range(2) for c in range(2)])
+               for s in range(2)]
+        expect = [[1.5, 0.5][t], chi0, math.sqrt(chi0_v / 2), xi, xi_e, mean(e_s),
+                  math.sqrt(var(e_s) / 2), mean(m_s), math.sqrt(var(m_s) / 2)]
+        for k, (a, b) in enumerate(zip(rows[t], expect)):
+            if math.isnan(b):
+                assert math.isnan(a)
+            else:
+                assert abs(a - b) <= 2e-6 * max(1.0, abs(b)) + 1e-11, (t, k, a, b)
+
+
+def test_obs_records_from_integer_fields():
+    """sg.obs_records turns U, K, W and plane counts into the reference's
+    float32 e, m, q_0, q_1 (eamsim.cu:320-325, 451-511)."""
+    L, R, H, N = 4, 2, 0.25, 64
+    rng = np.random.default_rng(9)
+    s = rng.choice([-1, 1], size=(R, L, L, L))     # [rho][z][y][x]
+    J = rng.choice([-1, 1], size=(3, L, L, L))
+    rec = np.zeros((1, 1, 64, 2 * R + 1 + 3 * L), dtype=np.int32)
+    for rho in range(R):
+        E = 0
+        for d, ax in enumerate((2, 1, 0)):
+            E -= (J[d] * s[rho] * np.roll(s[rho], -1, axis=ax)).sum()
+        U = (E + 3 * N) // 2
+        rec[0, 0, 0, rho] = U
+        rec[0, 0, 0, R + rho] = (s[rho] > 0).sum()
+    w = (s[0] != s[1]).astype(int)
+    rec[0, 0, 0, 2 * R] = w.sum()
+    rec[0, 0, 0, 2 * R + 1:2 * R + 1 + L] = w.sum(axis=(0, 1))
+    rec[0, 0, 0, 2 * R + 1 + L:2 * R + 1 + 2 * L] = w.sum(axis=(0, 2))
+    rec[0, 0, 0, 2 * R + 1 + 2 * L:] = w.sum(axis=(1, 2))
+    o = sg.obs_records(rec, L, R, H, 1)[0, 0]
+    q = s[0] * s[1]
+    k = 2 * math.pi / L
+    z, y, x = np.meshgrid(np.arange(L), np.arange(L), np.arange(L), indexing="ij")
+    for rho in range(R):
+        E = -sum((J[d] * s[rho] * np.roll(s[rho], -1, axis=ax)).sum()
+                 for d, ax in enumerate((2, 1, 0))) - H * s[rho].sum()
+        assert abs(o["e"][rho] - E / N) < 1e-6
+        assert abs(o["m"][rho] - s[rho].sum() / N) < 1e-6
+    assert abs(o["q_0"] - q.sum() / N) < 1e-6
+    for d, c in enumerate((x, y, z)):
+        assert abs(o["q_1_r"][d] - (q * np.cos(k * c)).sum() / N) < 1e-6
+        assert abs(o["q_1_i"][d] - (q * np.sin(k * c)).sum() / N) < 1e-6

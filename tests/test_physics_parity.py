@@ -1,0 +1,122 @@
+"""Statistical parity with the reference's published results.
+
+The reference's only result-pinning data are its res.dat tables
+(tests/golden/res, copied from Simulations/<L>/<H>-res.dat).  Bit-level
+parity with it is impossible (it draws cuRAND XORWOW numbers seeded by the
+clock), so these tests compare physics: high-temperature closed forms and the
+published values, within combined statistical errors.  The CPU tests use the
+oracle; the GPU tests run the product at the reference's own protocol."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import sg
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def read_res(L, H):
+    return np.loadtxt(os.path.join(REPO, "tests", "golden", "res", f"{L:02d}",
+                                   f"{H:06.4f}-res.dat"))
+
+
+def per_sample_means(records, L, R, H, n_samples, discard_half=True):
+    """records [m][group][temp][lane][field] -> per-sample time means of e,
+    m and q0^2 (shape [samples][temps]), the reference's first-half cut."""
+    rec = np.stack(records)
+    if discard_half:
+        rec = rec[len(rec) // 2:]
+    N = L ** 3
+    m_, G, T = rec.shape[0], rec.shape[1], rec.shape[2]
+    r = rec.transpose(0, 1, 3, 2, 4).reshape(m_, G * 64, T, -1)[:, :n_samples].astype(float)
+    U, K = r[..., :R], r[..., R:2 * R]
+    M = 2 * K - N
+    E = ((2 * U - 3 * N) - H * M) / N
+    e = E.mean(axis=(0, 3))
+    mag = (M / N).mean(axis=(0, 3))
+    q2 = (((N - 2 * r[..., 2 * R]) / N) ** 2).mean(axis=0)
+    return e, mag, q2
+
+
+def group_error(x, G):
+    """error of the mean over samples with whole 64-sample words as the
+    statistical unit (lanes of a word share random numbers)."""
+    if G < 2:
+        return x.std(axis=0, ddof=1) / math.sqrt(x.shape[0])
+    g = x[: G * 64].reshape(G, 64, *x.shape[1:]).mean(axis=1)
+    return g.std(axis=0, ddof=1) / math.sqrt(G)
+
+
+def test_high_temperature_oracle_matches_reference_and_closed_forms():
+    L, G, R = 8, 4, 2
+    betas = sg.REF_BETAS[-3:]                 # 0.1538, 0.1386, 0.125
+    o = oracle.OracleSim(L, G, R, betas, pt_every=16, meas_every=8,
+                         coupling_seed=11, spin_seed=12)
+    o.init()
+    o.run(4096)
+    recs = [o.meas(i) for i in range(o.n_meas())]
+    e, _, q2 = per_sample_means(recs, L, R, 0.0, G * 64)
+    e_err = group_error(e, G)
+    q2_err = group_error(q2, G)
+    ref = read_res(16, 0.0)[-3:]
+    N = L ** 3
+    for t, beta in enumerate(betas):
+        th = math.tanh(beta)
+        # e = -3 tanh(beta) + O(tanh^5): the reference itself sits on it
+        assert abs(e[:, t].mean() - ref[t, 5]) < 5 * math.hypot(e_err[t], ref[t, 6]) + 2e-5
+        assert abs(e[:, t].mean() + 3 * th) < 5 * e_err[t] + 3 * th ** 5
+        # N chi(0) = 1 + 6 t^2 + 30 t^4 + O(t^6), for L=8 here and L=16 there
+        nchi = N * q2[:, t].mean()
+        closed = 1 + 6 * th ** 2 + 30 * th ** 4
+        assert abs(nchi - closed) < 5 * N * q2_err[t] + 0.01
+        assert abs(16 ** 3 * ref[t, 1] - closed) < 5 * 16 ** 3 * ref[t, 2] + 0.01
+
+
+def test_paramagnetic_susceptibility_oracle():
+    """m / H = beta in the paramagnet (reference make-plots.py:211-219), and
+    the published L=16, H=1/16 value at beta = 0.125."""
+    L, G, R, H = 8, 4, 2, 0.0625
+    betas = sg.REF_BETAS[-2:]
+    o = oracle.OracleSim(L, G, R, betas, H=H, pt_every=16, meas_every=8,
+                         coupling_seed=21, spin_seed=22)
+    o.init()
+    o.run(4096)
+    recs = [o.meas(i) for i in range(o.n_meas())]
+    _, mag, _ = per_sample_means(recs, L, R, H, G * 64)
+    m_err = group_error(mag, G)
+    ref = read_res(16, H)[-2:]
+    for t, beta in enumerate(betas):
+        m = mag[:, t].mean()
+        assert abs(m - ref[t, 7]) < 5 * math.hypot(m_err[t], ref[t, 8])
+        assert abs(m / H - beta) < 5 * m_err[t] / H + 0.01
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("H", [0.0, 0.125])
+def test_reference_protocol_l16_matches_published_table(H):
+    """The reference's protocol at L=16 (its 24-temperature ladder, 2 copies,
+    PT every 16 sweeps, a measurement every 1024, first half discarded) on a
+    reduced budget: 2048 samples x 2^16 sweeps (the published tables use
+    2^22).  e must agree with the table at every temperature; chi(0) at the
+    temperatures equilibrated on this budget (beta <= 0.76)."""
+    L, R, S = 16, 2, 2048
+    sim = sg.Sim(L=L, n_samples=S, n_replicas=R, betas=sg.REF_BETAS, H=H, pt_every=16,
+                 meas_every=1024, form=sg.SG_FORM_BATCHED, coupling_seed=0x5EED0101,
+                 spin_seed=0x5EED0102)
+    sim.init_lattice()
+    sim.run(2 ** 16)
+    e, mag, q2 = per_sample_means([r for _, r in sim.records], L, R, H, S)
+    G = S // 64
+    e_err, q2_err = group_error(e, G), group_error(q2, G)
+    ref = read_res(L, H)
+    z_e = (e.mean(axis=0) - ref[:, 5]) / np.hypot(e_err, ref[:, 6])
+    z_q = (q2.mean(axis=0) - ref[:, 1]) / np.hypot(q2_err, ref[:, 2])
+    print("beta   z(e)   z(chi0)")
+    for t in range(24):
+        print(f"{ref[t, 0]:.4f} {z_e[t]:+.2f} {z_q[t]:+.2f}")
+    assert np.all(np.abs(z_e) < 5)
+    hot = ref[:, 0] <= 0.76
+    assert np.all(np.abs(z_q[hot]) < 5)
