@@ -32,14 +32,23 @@ def ladder(n_temps: int = 24) -> np.ndarray:
 
 def estimators(data: np.ndarray, L: int):
     """data: structured obs records [measurement][sample][temp]."""
-    n_dis = data.shape[1]
     chi_0 = np.average(data["q_0"] ** 2, axis=0)
+    re = np.average(data["q_1_r"] ** 2, axis=0)
+    im = np.average(data["q_1_i"] ** 2, axis=0)
+    e = np.average(data["e"], axis=(0, 3))
+    m = np.average(data["m"], axis=(0, 3))
+    return estimators_from_time_means(chi_0, re, im, e, m, L)
+
+
+def estimators_from_time_means(chi_0, re, im, e, m, L: int):
+    """Disorder averages from per-sample time means: chi_0 = <q0^2> [s][t],
+    re/im = <Re/Im q1^2> [s][t][3], e/m = <e>/<m> averaged over copies
+    [s][t].  Returns chi0, sigma, xi_L/L, sigma, e, sigma, m, sigma ([t])."""
+    n_dis = chi_0.shape[0]
     chi_0_m = np.average(chi_0, axis=0)
     chi_0_v = np.var(chi_0, axis=0)
     chi_0_e = np.sqrt(chi_0_v / n_dis)
 
-    re = np.average(data["q_1_r"] ** 2, axis=0)
-    im = np.average(data["q_1_i"] ** 2, axis=0)
     re_m, re_v = np.average(re, axis=0), np.var(re, axis=0)
     im_m, im_v = np.average(im, axis=0), np.var(im, axis=0)
     chi_1_m = np.average(re_m + im_m, axis=1)
@@ -53,9 +62,7 @@ def estimators(data: np.ndarray, L: int):
     xi_v = (xi / (2 * (ratio - 1))) ** 2 * ratio_v
     xi_e = np.sqrt(xi_v / n_dis)
 
-    e = np.average(data["e"], axis=(0, 3))
     e_m, e_e = np.average(e, axis=0), np.sqrt(np.var(e, axis=0) / n_dis)
-    m = np.average(data["m"], axis=(0, 3))
     m_m, m_e = np.average(m, axis=0), np.sqrt(np.var(m, axis=0) / n_dis)
     return chi_0_m, chi_0_e, xi, xi_e, e_m, e_e, m_m, m_e
 
@@ -72,14 +79,18 @@ def analyze_obs(sim_dir: str, L: int, H: float, n_samples: int = 2048, n_temps: 
     cols = estimators(data, L)
     beta = ladder(n_temps) if betas is None else np.asarray(betas)
     out = os.path.join(sub, f"{H:06.4f}-res.dat")
-    with open(out, "w") as fh:
-        fh.write(f"# tot_MC_steps : {2 * data.shape[0] * sweeps_per_record:,}\n")
-        for t in range(n_temps):
+    write_res(out, beta, cols, 2 * data.shape[0] * sweeps_per_record)
+    return out
+
+
+def write_res(path: str, beta, cols, tot_sweeps: int) -> None:
+    with open(path, "w") as fh:
+        fh.write(f"# tot_MC_steps : {tot_sweeps:,}\n")
+        for t in range(len(beta)):
             fh.write(f"{beta[t]:16.12f}")
             for c in cols:
                 fh.write(f"{c[t]:16.12f}")
             fh.write("\n")
-    return out
 
 
 if __name__ == "__main__":

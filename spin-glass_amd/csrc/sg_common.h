@@ -139,6 +139,19 @@ struct MeasDesc {
   uint64_t rho_stride;    // words between replicas
   uint64_t j_off;
   uint64_t out_off;       // int32 offset of the 64 records
+  uint64_t acc_off;       // ObsAcc index of lane 0
+  uint64_t cs_off;        // double offset of cos[L], sin[L] of 2 pi c / L
+};
+
+// Observable sums of one (group, temperature, lane) over all measurements:
+// exact integers (Q = N - 2W is the overlap times N; Q^4 needs 128 bits at
+// L = 48) and one FP64 sum for the k = 2 pi / L overlap.
+struct ObsAcc {
+  int64_t sU, sK;           // unsatisfied bonds, up spins: summed over replicas
+  int64_t sQ, sAbsQ;
+  uint64_t sQ2;
+  uint64_t sQ4lo, sQ4hi;
+  double sq1;               // sum of (|q1_x|^2 + |q1_y|^2 + |q1_z|^2) / 3
 };
 
 }  // namespace sg

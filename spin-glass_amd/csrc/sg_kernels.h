@@ -34,9 +34,11 @@ hipError_t launch_pt(uint64_t *spins, int32_t *energy, const PairDesc *pairs,
                      int npairs, const JobState *jobs,
                      unsigned long long *counters, hipStream_t st);
 
+// Measures every (group, temperature) of md: folds the measurement into the
+// ObsAcc sums and, with write_records, writes the int32 records to out.
 hipError_t launch_measure(int L, int R, const uint64_t *spins, const uint64_t *J,
-                          const MeasDesc *md, int nmd, int32_t *out, int nfield,
-                          hipStream_t st);
+                          const MeasDesc *md, int nmd, int32_t *out, int nfield, ObsAcc *acc,
+                          const double *cs, int write_records, hipStream_t st);
 
 hipError_t launch_init_spins(int L, uint64_t *spins, const LatDesc *descs,
                              int ndesc, hipStream_t st);

@@ -15,6 +15,61 @@ __device__ __forceinline__ uint64_t maj3(uint64_t a, uint64_t b, uint64_t c) {
   return (a & b) | (c & (a | b));
 }
 
+// v_bitop3_b32: any 3-input boolean function in one VALU op.  The 8-bit
+// immediate is the function applied to the operand patterns a = 0xF0,
+// b = 0xCC, c = 0xAA (bit 4a+2b+c of the immediate is f(a, b, c)).
+constexpr uint32_t kA = 0xF0, kB = 0xCC, kC = 0xAA;
+constexpr uint32_t kXor3 = (kA ^ kB ^ kC) & 0xFF;                      // a^b^c
+constexpr uint32_t kMaj3 = ((kA & kB) | (kA & kC) | (kB & kC)) & 0xFF;  // majority
+constexpr uint32_t kSel = ((kA & kB) | (~kA & kC)) & 0xFF;              // a ? b : c
+constexpr uint32_t kXorOr = (kA ^ (kB | kC)) & 0xFF;                    // a ^ (b|c)
+#define SG_BITOP3(a, b, c, imm) __builtin_amdgcn_bitop3_b32((a), (b), (c), (imm))
+
+// One Metropolis update of a 32-lane half word at H = 0: s is the centre
+// half, n[k]/j[k] the six neighbour and coupling halves, R0/R1/R2 all-ones
+// iff the site's random number is below the threshold for u = 0/1/2
+// unsatisfied bonds.  u >= 3 always flips.  18 VALU ops.
+__device__ __forceinline__ uint32_t metropolis32(uint32_t s, const uint32_t (&n)[6],
+                                                 const uint32_t (&j)[6], uint32_t R0,
+                                                 uint32_t R1, uint32_t R2) {
+  const uint32_t b0 = SG_BITOP3(s, n[0], j[0], kXor3);
+  const uint32_t b1 = SG_BITOP3(s, n[1], j[1], kXor3);
+  const uint32_t b2 = SG_BITOP3(s, n[2], j[2], kXor3);
+  const uint32_t b3 = SG_BITOP3(s, n[3], j[3], kXor3);
+  const uint32_t b4 = SG_BITOP3(s, n[4], j[4], kXor3);
+  const uint32_t b5 = SG_BITOP3(s, n[5], j[5], kXor3);
+  const uint32_t s1 = SG_BITOP3(b0, b1, b2, kXor3), c1 = SG_BITOP3(b0, b1, b2, kMaj3);
+  const uint32_t s2 = SG_BITOP3(b3, b4, b5, kXor3), c2 = SG_BITOP3(b3, b4, b5, kMaj3);
+  const uint32_t u0 = s1 ^ s2, c3 = s1 & s2;
+  const uint32_t u1 = SG_BITOP3(c1, c2, c3, kXor3), u2 = SG_BITOP3(c1, c2, c3, kMaj3);
+  const uint32_t lo = SG_BITOP3(u0, R1, R0, kSel);   // u = 1 or 0
+  const uint32_t hi = u0 | R2;                        // u = 3 or 2
+  const uint32_t mid = SG_BITOP3(u1, hi, lo, kSel);
+  return SG_BITOP3(s, u2, mid, kXorOr);               // s ^ (u >= 4 | mid)
+}
+
+// Philox4x32-10 with the key XORs as single 3-input ops (same numbers as
+// sg::philox).
+__device__ __forceinline__ u32x4 philox_dev(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                            uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int round = 0; round < 10; ++round) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = SG_BITOP3((uint32_t)(p1 >> 32), c1, k0, kXor3);
+    const uint32_t n2 = SG_BITOP3((uint32_t)(p0 >> 32), c3, k1, kXor3);
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  u32x4 r;
+  r.v[0] = c0; r.v[1] = c1; r.v[2] = c2; r.v[3] = c3;
+  return r;
+}
+
 template <int L>
 struct Geo {
   static constexpr int N = L * L * L;
@@ -190,37 +245,56 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_sweep_lds(
   }
   const uint32_t sw3 = (kTagMC << 28) | (d.rt & 0x0FFFFFFFu);
   __syncthreads();
+  // the thread's own four sites stay in registers between phases
+  uint64_t own[2][2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) own[c][q] = active ? lat[c * G::NH + tid + q * G::Q] : 0;
 
   for (int it = 0; it < nsweeps; ++it) {
-    const u32x4 r = philox((uint32_t)tid, n0 + (uint32_t)it, d.g, sw3, d.key0, d.key1);
+    const u32x4 r = philox_dev((uint32_t)tid, n0 + (uint32_t)it, d.g, sw3, d.key0, d.key1);
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       if (active) {
+        uint64_t nv[2][6];
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+          for (int k = 0; k < 6; ++k) nv[q][k] = lat[nb[q][c][k]];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           const int self = c * G::NH + tid + q * G::Q;
-          const uint64_t s = lat[self];
-          const uint64_t b0 = s ^ lat[nb[q][c][0]] ^ jw[q][c][0];
-          const uint64_t b1 = s ^ lat[nb[q][c][1]] ^ jw[q][c][1];
-          const uint64_t b2 = s ^ lat[nb[q][c][2]] ^ jw[q][c][2];
-          const uint64_t b3 = s ^ lat[nb[q][c][3]] ^ jw[q][c][3];
-          const uint64_t b4 = s ^ lat[nb[q][c][4]] ^ jw[q][c][4];
-          const uint64_t b5 = s ^ lat[nb[q][c][5]] ^ jw[q][c][5];
-          const uint64_t s1 = b0 ^ b1 ^ b2, c1 = maj3(b0, b1, b2);
-          const uint64_t s2 = b3 ^ b4 ^ b5, c2 = maj3(b3, b4, b5);
-          const uint64_t u0 = s1 ^ s2, c3 = s1 & s2;
-          const uint64_t u1 = c1 ^ c2 ^ c3, u2 = maj3(c1, c2, c3);
+          const uint64_t s = own[c][q];
           const uint32_t rr = r.v[2 * c + q];
-          uint64_t f;
+          uint64_t snew;
           if (!FIELD) {
             // u >= 3 always flips; u = 2, 1, 0 flip iff rr < T(u)
-            const uint64_t R0 = rr < tr.t[0][0] ? ~0ull : 0ull;
-            const uint64_t R1 = rr < tr.t[0][1] ? ~0ull : 0ull;
-            const uint64_t R2 = rr < tr.t[0][2] ? ~0ull : 0ull;
-            const uint64_t lo = (u0 & R1) | (~u0 & R0);
-            const uint64_t hi = u0 | R2;
-            f = u2 | (u1 & hi) | (~u1 & lo);
+            const uint32_t R0 = rr < tr.t[0][0] ? ~0u : 0u;
+            const uint32_t R1 = rr < tr.t[0][1] ? ~0u : 0u;
+            const uint32_t R2 = rr < tr.t[0][2] ? ~0u : 0u;
+            uint32_t nlo[6], nhi[6], jlo[6], jhi[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+              nlo[k] = (uint32_t)nv[q][k];
+              nhi[k] = (uint32_t)(nv[q][k] >> 32);
+              jlo[k] = (uint32_t)jw[q][c][k];
+              jhi[k] = (uint32_t)(jw[q][c][k] >> 32);
+            }
+            const uint32_t lo = metropolis32((uint32_t)s, nlo, jlo, R0, R1, R2);
+            const uint32_t hi = metropolis32((uint32_t)(s >> 32), nhi, jhi, R0, R1, R2);
+            snew = (uint64_t)lo | ((uint64_t)hi << 32);
           } else {
+            const uint64_t b0 = s ^ nv[q][0] ^ jw[q][c][0];
+            const uint64_t b1 = s ^ nv[q][1] ^ jw[q][c][1];
+            const uint64_t b2 = s ^ nv[q][2] ^ jw[q][c][2];
+            const uint64_t b3 = s ^ nv[q][3] ^ jw[q][c][3];
+            const uint64_t b4 = s ^ nv[q][4] ^ jw[q][c][4];
+            const uint64_t b5 = s ^ nv[q][5] ^ jw[q][c][5];
+            const uint64_t s1 = b0 ^ b1 ^ b2, c1 = maj3(b0, b1, b2);
+            const uint64_t s2 = b3 ^ b4 ^ b5, c2 = maj3(b3, b4, b5);
+            const uint64_t u0 = s1 ^ s2, c3 = s1 & s2;
+            const uint64_t u1 = c1 ^ c2 ^ c3, u2 = maj3(c1, c2, c3);
             int k0 = 0, k1 = 0;
 #pragma unroll
             for (int u = 0; u < 7; ++u) {
@@ -229,9 +303,10 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_sweep_lds(
             }
             const uint64_t f0 = count_ge(u2, u1, u0, k0);
             const uint64_t f1 = count_ge(u2, u1, u0, k1);
-            f = (s & f1) | (~s & f0);
+            snew = s ^ ((s & f1) | (~s & f0));
           }
-          lat[self] = s ^ f;
+          own[c][q] = snew;
+          lat[self] = snew;
         }
       }
       __syncthreads();
@@ -330,7 +405,10 @@ template <int L>
 __global__ __launch_bounds__(256) void k_measure(const uint64_t *__restrict__ spins,
                                                  const uint64_t *__restrict__ Jg,
                                                  const MeasDesc *__restrict__ md, int R,
-                                                 int32_t *__restrict__ out, int nfield) {
+                                                 int32_t *__restrict__ out, int nfield,
+                                                 ObsAcc *__restrict__ acc,
+                                                 const double *__restrict__ cs,
+                                                 int write_records) {
   constexpr int N = L * L * L;
   __shared__ int32_t sx[L][64], sy[L][64], sz[L][64];
   __shared__ int32_t sU[8][64], sK[8][64], sW[64];
@@ -391,6 +469,43 @@ __global__ __launch_bounds__(256) void k_measure(const uint64_t *__restrict__ sp
     atomicAdd(&sW[lane], wt);
   }
   __syncthreads();
+  if (tid < 64) {
+    // fold this measurement into the lane's running sums (one owner per lane)
+    ObsAcc a = acc[d.acc_off + tid];
+    for (int rho = 0; rho < R; ++rho) {
+      a.sU += sU[rho][tid];
+      a.sK += sK[rho][tid];
+    }
+    if (R >= 2) {
+#pragma clang fp contract(off)
+      const int64_t Q = (int64_t)N - 2 * (int64_t)sW[tid];
+      const uint64_t Q2 = (uint64_t)(Q * Q);
+      const unsigned __int128 Q4 = (unsigned __int128)Q2 * Q2;
+      const unsigned __int128 s4 =
+          (((unsigned __int128)a.sQ4hi << 64) | a.sQ4lo) + Q4;
+      a.sQ += Q;
+      a.sAbsQ += Q < 0 ? -Q : Q;
+      a.sQ2 += Q2;
+      a.sQ4lo = (uint64_t)s4;
+      a.sQ4hi = (uint64_t)(s4 >> 64);
+      const double *cw = cs + d.cs_off, *sw = cw + L;
+      double q1 = 0.0;
+      for (int dir = 0; dir < 3; ++dir) {
+        double re = 0.0, im = 0.0;
+        for (int c = 0; c < L; ++c) {
+          const double w = (double)(dir == 0 ? sx[c][tid] : dir == 1 ? sy[c][tid] : sz[c][tid]);
+          re += cw[c] * w;
+          im += sw[c] * w;
+        }
+        re *= -2.0 / N;
+        im *= -2.0 / N;
+        q1 += re * re + im * im;
+      }
+      a.sq1 += q1 / 3.0;
+    }
+    acc[d.acc_off + tid] = a;
+  }
+  if (!write_records) return;
   int32_t *o = out + d.out_off;
   for (int i = tid; i < 64 * nfield; i += 256) {
     const int l = i / nfield, f = i % nfield;
@@ -486,15 +601,15 @@ hipError_t launch_pt(uint64_t *spins, int32_t *energy, const PairDesc *pairs, in
 }
 
 hipError_t launch_measure(int L, int R, const uint64_t *spins, const uint64_t *J,
-                          const MeasDesc *md, int nmd, int32_t *out, int nfield,
-                          hipStream_t st) {
+                          const MeasDesc *md, int nmd, int32_t *out, int nfield, ObsAcc *acc,
+                          const double *cs, int write_records, hipStream_t st) {
   if (nmd <= 0) return hipSuccess;
   if (R < 1 || R > 8) return hipErrorInvalidValue;
   switch (L) {
 #define X(l)                                                                            \
   case l:                                                                               \
     hipLaunchKernelGGL(k_measure<l>, dim3(nmd), dim3(256), 0, st, spins, J, md, R, out, \
-                       nfield);                                                         \
+                       nfield, acc, cs, write_records);                                 \
     return hipGetLastError();
     SG_ALL_SIZES(X)
 #undef X

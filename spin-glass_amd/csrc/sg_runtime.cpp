@@ -26,13 +26,6 @@ namespace {
 
 thread_local std::string g_create_error;
 
-struct SampleAcc {         // per (sample, temperature)
-  int64_t n = 0;
-  double sum_e = 0, sum_m = 0, sum_absq = 0, sum_q1sq = 0;
-  int64_t sum_Q = 0;
-  unsigned __int128 sum_Q2 = 0, sum_Q4 = 0;
-};
-
 struct Fail {
   sg_status code;
   std::string msg;
@@ -86,12 +79,21 @@ struct sg_handle {
   MeasDesc *d_meas = nullptr;
   JobState *d_jobs = nullptr;
   unsigned long long *d_cnt = nullptr;
+  ObsAcc *d_acc = nullptr;        // [group][temp][lane]
+  double *d_cs = nullptr;         // cos, sin of 2 pi c / L
   int npairs = 0, nfield = 0;
 
-  std::vector<int32_t> h_rec;
-  std::vector<SampleAcc> acc;     // [sample][t]
+  // records for the sink: the measurement kernel writes d_rec, an async copy
+  // moves it to one of kRing pinned buffers, and the sink runs once the copy
+  // has landed (while later intervals already run on the device)
+  static constexpr int kRing = 3;
+  int32_t *h_ring[kRing] = {nullptr, nullptr, nullptr};
+  hipEvent_t ring_ev[kRing] = {nullptr, nullptr, nullptr};
+  struct PendingRec { int slot; int64_t sweep; };
+  std::vector<PendingRec> rec_queue;
+  int ring_next = 0;
+  size_t rec_count = 0;
   std::vector<uint64_t> swap_att; // [T-1]
-  std::vector<double> cosw, sinw; // [L]
   bool initialized = false, energy_valid = false;
   int64_t sweep = 0, n_meas = 0;
   double kernel_ms = 0, flips = 0;
@@ -152,8 +154,16 @@ struct sg_handle {
     graph = nullptr;
     hipFree(d_spins); hipFree(d_J); hipFree(d_thr); hipFree(d_energy); hipFree(d_rec);
     hipFree(d_lat); hipFree(d_pairs); hipFree(d_meas); hipFree(d_jobs); hipFree(d_cnt);
+    hipFree(d_acc); hipFree(d_cs);
     d_spins = d_J = nullptr; d_thr = nullptr; d_energy = d_rec = nullptr; d_lat = nullptr;
     d_pairs = nullptr; d_meas = nullptr; d_jobs = nullptr; d_cnt = nullptr;
+    d_acc = nullptr; d_cs = nullptr;
+    for (int i = 0; i < kRing; ++i) {
+      if (h_ring[i]) hipHostFree(h_ring[i]);
+      if (ring_ev[i]) hipEventDestroy(ring_ev[i]);
+      h_ring[i] = nullptr;
+      ring_ev[i] = nullptr;
+    }
     for (auto &p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
     pending.clear();
     for (auto e : ev_pool) hipEventDestroy(e);
@@ -257,18 +267,35 @@ struct sg_handle {
         d.rho_stride = (uint64_t)T * N;
         d.j_off = (uint64_t)g * 3 * N;
         d.out_off = ((uint64_t)g * T + t) * 64 * nfield;
+        d.acc_off = ((uint64_t)g * T + t) * 64;
+        d.cs_off = 0;
       }
     HIPCHK(hipMemcpy(d_meas, md.data(), sizeof(MeasDesc) * md.size(), hipMemcpyHostToDevice));
 
-    h_rec.resize((size_t)G * T * 64 * nfield);
-    acc.assign((size_t)S * T, SampleAcc());
-    swap_att.assign((size_t)std::max(T - 1, 1), 0);
-    cosw.resize(L); sinw.resize(L);
+    dev_alloc(&d_acc, (size_t)G * T * 64);
+    HIPCHK(hipMemset(d_acc, 0, sizeof(ObsAcc) * (size_t)G * T * 64));
+    std::vector<double> cs(2 * L);
     for (int c = 0; c < L; ++c) {
-      cosw[c] = std::cos(2.0 * M_PI * c / L);
-      sinw[c] = std::sin(2.0 * M_PI * c / L);
+      cs[c] = std::cos(2.0 * M_PI * c / L);
+      cs[L + c] = std::sin(2.0 * M_PI * c / L);
     }
+    dev_alloc(&d_cs, (size_t)2 * L);
+    HIPCHK(hipMemcpy(d_cs, cs.data(), sizeof(double) * 2 * L, hipMemcpyHostToDevice));
+    rec_count = (size_t)G * T * 64 * nfield;
+    swap_att.assign((size_t)std::max(T - 1, 1), 0);
     reset_counters();
+  }
+
+  void ensure_ring() {
+    if (h_ring[0]) return;
+    for (int i = 0; i < kRing; ++i) {
+      hipError_t e = hipHostMalloc((void **)&h_ring[i], sizeof(int32_t) * rec_count);
+      if (e != hipSuccess) {
+        h_ring[i] = nullptr;
+        throw Fail{SG_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e)};
+      }
+      HIPCHK(hipEventCreateWithFlags(&ring_ev[i], hipEventDisableTiming));
+    }
   }
 
   void build_thresholds(double b, ThrRec &r) const {
@@ -309,7 +336,7 @@ struct sg_handle {
     HIPCHK(hipStreamSynchronize(st));
     sweep = 0;
     n_meas = 0;
-    acc.assign(acc.size(), SampleAcc());
+    HIPCHK(hipMemsetAsync(d_acc, 0, sizeof(ObsAcc) * (size_t)G * T * 64, st));
     HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * std::max(T - 1, 1), st));
     std::fill(swap_att.begin(), swap_att.end(), 0);
     reset_counters();
@@ -431,8 +458,11 @@ struct sg_handle {
       energy_valid = want_energy;
       flips += (double)seg * N * 64.0 * G * R * T;
       remaining -= seg;
+      // records of earlier measurements whose copies have landed
+      if (!rec_queue.empty()) drain_records(sink, user, 0);
       if (meas_every > 0 && next % meas_every == 0) measure(sink, user);
     }
+    drain_records(sink, user, -1);
     HIPCHK(hipEventRecord(ev1, st));
     HIPCHK(hipEventSynchronize(ev1));
     float ms = 0;
@@ -441,76 +471,74 @@ struct sg_handle {
     collect_profile();
   }
 
+  // The measurement folds itself into the device sums (ObsAcc); records leave
+  // the device only when a sink wants them.
   void measure(sg_obs_sink sink, void *user) {
+    const bool want = sink != nullptr;
     timed(2, 0.0, [&] {
-      HIPCHK(launch_measure(L, R, d_spins, d_J, d_meas, G * T, d_rec, nfield, st));
+      HIPCHK(launch_measure(L, R, d_spins, d_J, d_meas, G * T, d_rec, nfield, d_acc, d_cs,
+                            want ? 1 : 0, st));
     });
-    HIPCHK(hipMemcpyAsync(h_rec.data(), d_rec, sizeof(int32_t) * h_rec.size(),
-                          hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    accumulate(h_rec.data());
     ++n_meas;
-    if (sink) sink(user, sweep, h_rec.data(), (int64_t)G * T * 64, nfield);
+    if (!want) return;
+    ensure_ring();
+    // a ring slot is reused only after its sink call has run
+    if ((int)rec_queue.size() == kRing) drain_records(sink, user, 1);
+    const int slot = ring_next;
+    ring_next = (ring_next + 1) % kRing;
+    HIPCHK(hipMemcpyAsync(h_ring[slot], d_rec, sizeof(int32_t) * rec_count,
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(ring_ev[slot], st));
+    rec_queue.push_back(PendingRec{slot, sweep});
   }
 
-  void accumulate(const int32_t *rec) {
-    for (int g = 0; g < G; ++g)
-      for (int t = 0; t < T; ++t)
-        for (int l = 0; l < 64; ++l) {
-          const int sample = g * 64 + l;
-          if (sample >= S) continue;
-          const int32_t *f = rec + (((size_t)g * T + t) * 64 + l) * nfield;
-          SampleAcc &a = acc[(size_t)sample * T + t];
-          double se = 0, sm = 0;
-          for (int rho = 0; rho < R; ++rho) {
-            const double M = (double)(2 * f[R + rho] - N);
-            se += ((double)(2 * f[rho] - 3 * N) - H * M) / N;
-            sm += M / N;
-          }
-          a.sum_e += se / R;
-          a.sum_m += sm / R;
-          if (R >= 2) {
-            const int64_t Q = (int64_t)N - 2 * (int64_t)f[2 * R];
-            a.sum_Q += Q;
-            const unsigned __int128 Q2 = (unsigned __int128)(Q * Q);
-            a.sum_Q2 += Q2;
-            a.sum_Q4 += Q2 * Q2;
-            a.sum_absq += (double)(Q < 0 ? -Q : Q) / N;
-            double q1 = 0;
-            for (int dir = 0; dir < 3; ++dir) {
-              const int32_t *W = f + 2 * R + 1 + dir * L;
-              double re = 0, im = 0;
-              for (int c = 0; c < L; ++c) {
-                re += cosw[c] * W[c];
-                im += sinw[c] * W[c];
-              }
-              re *= -2.0 / N;
-              im *= -2.0 / N;
-              q1 += re * re + im * im;
-            }
-            a.sum_q1sq += q1 / 3.0;
-          }
-          a.n += 1;
-        }
+  // Runs the sink on copied records in measurement order: all of them with
+  // at_least < 0, otherwise at least that many plus any already landed.
+  void drain_records(sg_obs_sink sink, void *user, int at_least) {
+    size_t i = 0;
+    for (; i < rec_queue.size(); ++i) {
+      const PendingRec &p = rec_queue[i];
+      const bool must = at_least < 0 || (int)i < at_least;
+      if (!must) {
+        const hipError_t q = hipEventQuery(ring_ev[p.slot]);
+        if (q == hipErrorNotReady) break;
+        HIPCHK(q);
+      } else {
+        HIPCHK(hipEventSynchronize(ring_ev[p.slot]));
+      }
+      if (sink) sink(user, p.sweep, h_ring[p.slot], (int64_t)G * T * 64, nfield);
+    }
+    rec_queue.erase(rec_queue.begin(), rec_queue.begin() + (long)i);
   }
 
-  void results(double *res) const {
+  void read_acc(std::vector<ObsAcc> &a) {
+    a.resize((size_t)G * T * 64);
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpy(a.data(), d_acc, sizeof(ObsAcc) * a.size(), hipMemcpyDeviceToHost));
+  }
+
+  // Per temperature, time means per sample from the exact sums, then the
+  // mean over the active samples.
+  void results(double *res) {
+    std::vector<ObsAcc> acc;
+    read_acc(acc);
     const double Nd = N;
+    const double n = (double)n_meas;
     for (int t = 0; t < T; ++t) {
       double e = 0, m = 0, aq = 0, q2 = 0, q4 = 0, c1 = 0;
-      int64_t nm = 0;
-      for (int s = 0; s < S; ++s) {
-        const SampleAcc &a = acc[(size_t)s * T + t];
-        if (a.n == 0) continue;
-        const double n = (double)a.n;
-        e += a.sum_e / n;
-        m += a.sum_m / n;
-        aq += a.sum_absq / n;
-        q2 += (double)a.sum_Q2 / n / (Nd * Nd);
-        q4 += (double)a.sum_Q4 / n / (Nd * Nd * Nd * Nd);
-        c1 += a.sum_q1sq / n;
-        nm = a.n;
+      for (int s = 0; s < S && n_meas > 0; ++s) {
+        const ObsAcc &a = acc[((size_t)(s / 64) * T + t) * 64 + (s % 64)];
+        const double U = (double)a.sU / (n * R), K = (double)a.sK / (n * R);
+        const double M = 2.0 * K - Nd;
+        e += ((2.0 * U - 3.0 * Nd) - H * M) / Nd;
+        m += M / Nd;
+        aq += (double)a.sAbsQ / n / Nd;
+        q2 += (double)a.sQ2 / n / (Nd * Nd);
+        const unsigned __int128 q4s = ((unsigned __int128)a.sQ4hi << 64) | a.sQ4lo;
+        q4 += (double)q4s / n / (Nd * Nd * Nd * Nd);
+        c1 += a.sq1 / n;
       }
+      const int64_t nm = n_meas;
       double *r = res + (size_t)t * SG_NRES;
       const double ns = S;
       e /= ns; m /= ns; aq /= ns; q2 /= ns; q4 /= ns; c1 /= ns;
@@ -543,7 +571,7 @@ struct sg_handle {
   size_t ck_size() const {
     return sizeof(CkHeader) + sizeof(double) * T + sizeof(uint64_t) * n_lat() * N +
            sizeof(uint64_t) * (size_t)G * 3 * N + sizeof(uint64_t) * 2 * std::max(T - 1, 1) +
-           sizeof(SampleAcc) * acc.size() + sizeof(uint64_t);
+           sizeof(ObsAcc) * (size_t)G * T * 64 + sizeof(uint64_t);
   }
 
   void save(void *buf, size_t *size) {
@@ -555,7 +583,7 @@ struct sg_handle {
     CkHeader h;
     std::memset(&h, 0, sizeof(h));
     std::memcpy(h.magic, "SGCHKPT1", 8);
-    h.version = 1; h.L = L; h.G = G; h.R = R; h.T = T; h.S = S;
+    h.version = 2; h.L = L; h.G = G; h.R = R; h.T = T; h.S = S;
     h.pt_every = pt_every; h.meas_every = meas_every; h.H = H; h.cseed = cseed; h.sseed = sseed;
     h.sweep = sweep; h.n_meas = n_meas;
     h.payload_bytes = need - sizeof(CkHeader) - sizeof(uint64_t);
@@ -568,7 +596,9 @@ struct sg_handle {
     HIPCHK(hipMemcpy(cnt.data(), d_cnt, sizeof(uint64_t) * cnt.size(), hipMemcpyDeviceToHost));
     std::memcpy(p, cnt.data(), sizeof(uint64_t) * cnt.size()); p += sizeof(uint64_t) * cnt.size();
     std::memcpy(p, swap_att.data(), sizeof(uint64_t) * cnt.size()); p += sizeof(uint64_t) * cnt.size();
-    std::memcpy(p, acc.data(), sizeof(SampleAcc) * acc.size()); p += sizeof(SampleAcc) * acc.size();
+    std::vector<ObsAcc> acc;
+    read_acc(acc);
+    std::memcpy(p, acc.data(), sizeof(ObsAcc) * acc.size()); p += sizeof(ObsAcc) * acc.size();
     const uint64_t ck = fnv1a(buf, need - sizeof(uint64_t));
     std::memcpy(p, &ck, sizeof(ck));
     *size = need;
@@ -580,7 +610,7 @@ struct sg_handle {
     const char *p = (const char *)buf;
     CkHeader h;
     std::memcpy(&h, p, sizeof(h));
-    if (std::memcmp(h.magic, "SGCHKPT1", 8) != 0 || h.version != 1)
+    if (std::memcmp(h.magic, "SGCHKPT1", 8) != 0 || h.version != 2)
       throw Fail{SG_ERR_FORMAT, "checkpoint magic/version mismatch"};
     uint64_t ck;
     std::memcpy(&ck, p + need - sizeof(uint64_t), sizeof(ck));
@@ -601,7 +631,7 @@ struct sg_handle {
     const size_t nc = std::max(T - 1, 1);
     HIPCHK(hipMemcpy(d_cnt, p, sizeof(uint64_t) * nc, hipMemcpyHostToDevice)); p += sizeof(uint64_t) * nc;
     std::memcpy(swap_att.data(), p, sizeof(uint64_t) * nc); p += sizeof(uint64_t) * nc;
-    std::memcpy(acc.data(), p, sizeof(SampleAcc) * acc.size());
+    HIPCHK(hipMemcpy(d_acc, p, sizeof(ObsAcc) * (size_t)G * T * 64, hipMemcpyHostToDevice));
     sweep = h.sweep;
     n_meas = h.n_meas;
     reset_counters();

@@ -168,7 +168,7 @@ class Sim:
                 self.records.append((sweep, arr))
             if sink is not None:
                 sink(sweep, arr)
-        cb = SINK(_cb)
+        cb = SINK(_cb) if (self.keep_records or sink is not None) else SINK()
         self._c(load_library().sg_run(self._h, int(n_sweeps), cb, None))
 
     def spins(self):

@@ -117,3 +117,52 @@ def test_checkpoint_resume_is_bit_exact():
         b.load_checkpoint_bytes(bytes(bad))
     with pytest.raises(sg.SGError, match="size"):
         b.load_checkpoint_bytes(blob[:-8])
+
+
+def test_product_reproduces_canonical_digests():
+    import hashlib
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "canonical.json")))
+    for name, g in gold.items():
+        p = g["params"]
+        sim = sg.Sim(L=p["L"], n_samples=64 * p["groups"], n_replicas=p["replicas"],
+                     betas=p["betas"], H=p["H"], coupling_seed=CSEED, spin_seed=SSEED,
+                     pt_every=p["pt_every"], meas_every=p["meas_every"])
+        sim.init_lattice()
+        sim.run(p["sweeps"])
+        dig = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+        assert dig(sim.couplings()) == g["couplings"], name
+        assert dig(sim.spins()) == g["spins"], name
+        assert [dig(r) for _, r in sim.records] == g["records"], name
+
+
+def test_cli_resume_equals_straight_run(tmp_path):
+    """csg <sim_dir> <H> twice (the second resumes from the checkpoint)
+    appends exactly the obs.bin records of one run twice as long; the log
+    file is removed on success; analysis.py turns obs.bin into res.dat."""
+    import os
+    import subprocess
+    import analysis
+    csg = os.path.join(os.path.dirname(sg.__file__), "bin", "csg")
+    flags = ["--L", "8", "--samples", "96", "--meas-every", "64", "--temps", "6",
+             "--tmin", "0.5", "--tmax", "3.0", "--pt-every", "4"]
+    a, b = tmp_path / "a" / "08", tmp_path / "b" / "08"
+    a.mkdir(parents=True)
+    b.mkdir(parents=True)
+    for _ in range(2):
+        subprocess.run([csg, str(a), "0.25", "--sweeps", "512", *flags], check=True,
+                       cwd=tmp_path, capture_output=True)
+    subprocess.run([csg, str(b), "0.25", "--sweeps", "1024", *flags], check=True,
+                   cwd=tmp_path, capture_output=True)
+    ra = (a / "0.2500-obs.bin").read_bytes()
+    rb = (b / "0.2500-obs.bin").read_bytes()
+    assert len(ra) == 16 * 96 * 6 * 44
+    assert ra == rb
+    assert not list(tmp_path.glob("*.log"))
+    out = analysis.analyze_obs(str(tmp_path / "a"), 8, 0.25, n_samples=96, n_temps=6,
+                               betas=1.0 / (0.5 * 6.0 ** (np.arange(6) / 5)), sweeps_per_record=64)
+    rows = np.loadtxt(out)
+    assert rows.shape == (6, 9)
+    assert np.all(rows[:, 5] < 0)            # e < 0 at every temperature
+    assert np.all(np.diff(rows[:, 5]) > 0)   # and rises with T

