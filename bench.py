@@ -31,7 +31,10 @@ T_MIN, T_MAX = 0.2, 2.0
 PT_EVERY, MEAS_EVERY = 16, 1000
 SAMPLES_BATCHED, SAMPLES_PLAIN = 2048, 64
 W_ALG = 1.22          # int32 VALU ops per spin-flip attempt (algorithmic)
-VALU_PEAK_TOPS = 39.3  # 256 CU x 64 lanes x 2.4 GHz, int32 ops
+# int32 VALU peak: 256 CU x 4 SIMD x 32 lanes x 2.4 GHz.  A v_bitop3_b32
+# throughput probe on the MI355X sustains 70.6 T lane-ops/s (90 %), so the
+# 16-lane-per-SIMD figure (39.3) understates the hardware by 2x.
+VALU_PEAK_TOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
 
