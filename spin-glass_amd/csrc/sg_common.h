@@ -133,6 +133,15 @@ struct PairDesc {
   uint32_t parity;        // t & 1: the pair is active on steps with b & 1 == parity
 };
 
+// One work-group task of the sweep kernel: lattice descriptor indices (the
+// second is 0xFFFFFFFF for a single lattice) and, when the two form a PT
+// pair (slots t and t+1), the PairDesc index.
+struct TaskDesc {
+  uint32_t lat[2];
+  uint32_t pair;
+  uint32_t pad;
+};
+
 // One (group, temperature) measured across its replicas.
 struct MeasDesc {
   uint64_t spin_off;      // replica 0 lattice

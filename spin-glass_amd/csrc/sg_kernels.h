@@ -16,17 +16,15 @@ struct JobState {
 // largest L whose lattice the LDS-resident sweep kernel holds
 bool lds_kernel_supported(int L);
 
-// k sweeps of every lattice in descs (one work-group per lattice); with
-// want_energy the per-lane U and K of the final state go to the energy slots.
-hipError_t launch_sweep(int L, bool field, uint64_t *spins, const uint64_t *J,
-                        const LatDesc *descs, int ndesc, const ThrRec *thr,
-                        const JobState *jobs, int nsweeps, int want_energy,
-                        int32_t *energy, hipStream_t st);
-
-// energy of every lattice (no sweep)
-hipError_t launch_energy(int L, uint64_t *spins, const uint64_t *J,
-                         const LatDesc *descs, int ndesc, int32_t *energy,
-                         hipStream_t st);
+// One work-group per task (one lattice, or the lattices of slots t and t+1
+// of one group and replica): with do_pt, the task's PT pair is decided and
+// exchanged first; then nsweeps sweeps; want_energy = 1 writes the per-lane
+// U of the final states to the energy slots, 2 writes U and K.
+hipError_t launch_block(int L, bool field, uint64_t *spins, const uint64_t *J,
+                        const LatDesc *descs, const TaskDesc *tasks, int ntasks,
+                        const PairDesc *pairs, const ThrRec *thr, const JobState *jobs,
+                        int do_pt, int nsweeps, int want_energy, int32_t *energy,
+                        unsigned long long *counters, hipStream_t st);
 
 // PT step of every pair whose parity matches the job's current step; pairs
 // of the other parity return at once.

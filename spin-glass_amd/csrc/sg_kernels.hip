@@ -117,34 +117,56 @@ __device__ __forceinline__ void vc_add1(uint64_t (&A)[NP], uint64_t w) {
   }
 }
 
+// DPP source fetch of a 64-bit value (two 32-bit DPP moves); lanes whose
+// source is outside the row, or whose row is not in ROW_MASK, read 0.
+// GFX9 DPP controls: row_shr:n = 0x110 + n, row_bcast:15 = 0x142,
+// row_bcast:31 = 0x143.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, ROW_MASK, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, ROW_MASK, 0xF, true);
+  return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
+}
+
+// A (NP bit planes) += the DPP-fetched A of another lane; NP + 1 planes out.
+template <int NP, int CTRL, int ROW_MASK, int NA>
+__device__ __forceinline__ void dpp_add(uint64_t (&A)[NA]) {
+  static_assert(NP < NA, "plane array too small");
+  uint64_t B[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) B[p] = dpp64<CTRL, ROW_MASK>(A[p]);
+  uint64_t carry = 0;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const uint64_t a = A[p];
+    A[p] = a ^ B[p] ^ carry;
+    carry = maj3(a, B[p], carry);
+  }
+  A[NP] = carry;
+}
+
 // Sum the bit-sliced per-thread counters of a work-group into one integer per
-// lane (sample).  A holds P0 planes on entry; the butterfly adds lanes l and
-// l^2^k (6 levels, +1 plane each) so every lane holds its wave's total; the
-// per-wave totals then meet in wave 0 through LDS and a second 6-level
-// butterfly (lanes >= NW contribute zero), after which every lane of wave 0
-// holds the work-group total and lane l extracts sample l.  Result valid in
-// wave 0.  red must hold NW*24 words.  Every thread must call it.
+// lane (sample).  A0 holds P0 planes.  Within a wave, a DPP tree (row_shr 1,
+// 2, 4, 8, then row_bcast 15 and 31) leaves the wave total in lane 63 (P0+6
+// planes); the wave totals meet in wave 0 through LDS, a second row_shr tree
+// leaves the work-group total in lane 15 (P0+10 planes), and every lane
+// reads those planes back (readlane) and extracts its own bit.  Result valid
+// in wave 0.  red must hold NW*24 words; NW <= 16.  Every thread calls it.
 template <int P0, int NW>
 __device__ int32_t wg_sum_planes(const uint64_t (&A0)[P0], uint64_t *red) {
-  constexpr int PW = P0 + 6, PT = P0 + 12;
+  static_assert(NW <= 16, "at most 16 waves");
+  constexpr int PW = P0 + 6, PT = P0 + 10;
   uint64_t A[PT];
 #pragma unroll
   for (int p = 0; p < P0; ++p) A[p] = A0[p];
-#pragma unroll
-  for (int lvl = 0; lvl < 6; ++lvl) {
-    const int np = P0 + lvl;
-    uint64_t carry = 0;
-#pragma unroll
-    for (int p = 0; p < np; ++p) {
-      const uint64_t b = __shfl_xor(A[p], 1 << lvl);
-      const uint64_t a = A[p];
-      A[p] = a ^ b ^ carry;
-      carry = maj3(a, b, carry);
-    }
-    A[np] = carry;
-  }
+  dpp_add<P0 + 0, 0x111, 0xF>(A);
+  dpp_add<P0 + 1, 0x112, 0xF>(A);
+  dpp_add<P0 + 2, 0x114, 0xF>(A);
+  dpp_add<P0 + 3, 0x118, 0xF>(A);
+  dpp_add<P0 + 4, 0x142, 0xA>(A);
+  dpp_add<P0 + 5, 0x143, 0xC>(A);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
+  if (lane == 63) {
 #pragma unroll
     for (int p = 0; p < PW; ++p) red[wave * 24 + p] = A[p];
   }
@@ -153,21 +175,17 @@ __device__ int32_t wg_sum_planes(const uint64_t (&A0)[P0], uint64_t *red) {
   if (wave == 0) {
 #pragma unroll
     for (int p = 0; p < PW; ++p) A[p] = lane < NW ? red[lane * 24 + p] : 0;
+    dpp_add<PW + 0, 0x111, 0xF>(A);
+    dpp_add<PW + 1, 0x112, 0xF>(A);
+    dpp_add<PW + 2, 0x114, 0xF>(A);
+    dpp_add<PW + 3, 0x118, 0xF>(A);
 #pragma unroll
-    for (int lvl = 0; lvl < 6; ++lvl) {
-      const int np = PW + lvl;
-      uint64_t carry = 0;
-#pragma unroll
-      for (int p = 0; p < np; ++p) {
-        const uint64_t b = __shfl_xor(A[p], 1 << lvl);
-        const uint64_t a = A[p];
-        A[p] = a ^ b ^ carry;
-        carry = maj3(a, b, carry);
-      }
-      A[np] = carry;
+    for (int p = 0; p < PT; ++p) {
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)A[p], 15);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(A[p] >> 32), 15);
+      const uint64_t tot = (uint64_t)lo | ((uint64_t)hi << 32);
+      v |= (int32_t)((tot >> lane) & 1) << p;
     }
-#pragma unroll
-    for (int p = 0; p < PT; ++p) v |= (int32_t)((A[p] >> lane) & 1) << p;
   }
   __syncthreads();
   return v;
@@ -188,27 +206,119 @@ __device__ __forceinline__ uint64_t count_ge(uint64_t u2, uint64_t u1, uint64_t 
 }
 
 // ------------------------------------------------------------- sweep kernel
-// One work-group per lattice.  Thread j owns the Philox quad j: black and
-// white sites at h = j and h = j + N/4.  Per sweep it draws one Philox block
+// One work-group per task: one lattice, or two lattices of adjacent
+// temperature slots t, t+1 of the same (group, replica).  Both share the
+// couplings and neighbour indices held in VGPRs.  When the launch starts at
+// a PT step and the task is that step's pair, the group first decides the
+// 64 lane swaps (wave 0, one Philox word per lane) and exchanges the masked
+// bits in LDS; then both lattices are swept nsweeps times in LDS; then the
+// per-lane U (and K) of the final states are reduced for the next PT step.
+// Thread j owns the Philox quad j of each lattice: black and white sites at
+// h = j and h = j + N/4; per sweep it draws one Philox block per lattice
 // (r0, r1 for its black sites, r2, r3 for its white sites).
+// Reference counterparts: perform_MC_steps and perform_PT_shuffle,
+// Program/src/eamsim.cu:112-330.
 template <int L, bool FIELD>
-__global__ __launch_bounds__(Geo<L>::NT) void k_sweep_lds(
-    uint64_t *__restrict__ spins, const uint64_t *__restrict__ Jg,
-    const LatDesc *__restrict__ descs, const ThrRec *__restrict__ thr,
-    const JobState *__restrict__ jobs, int nsweeps, int want_energy,
-    int32_t *__restrict__ energy) {
-  using G = Geo<L>;
-  __shared__ uint64_t lat[G::N];
-  __shared__ uint64_t red[G::NW * 24];
+__device__ __forceinline__ void update_site(uint64_t &s, const uint64_t (&nv)[6],
+                                            const uint64_t (&jw)[6], uint32_t rr,
+                                            const ThrRec &tr) {
+  if (!FIELD) {
+    // u >= 3 always flips; u = 2, 1, 0 flip iff rr < T(u)
+    const uint32_t R0 = rr < tr.t[0][0] ? ~0u : 0u;
+    const uint32_t R1 = rr < tr.t[0][1] ? ~0u : 0u;
+    const uint32_t R2 = rr < tr.t[0][2] ? ~0u : 0u;
+    uint32_t nlo[6], nhi[6], jlo[6], jhi[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      nlo[k] = (uint32_t)nv[k];
+      nhi[k] = (uint32_t)(nv[k] >> 32);
+      jlo[k] = (uint32_t)jw[k];
+      jhi[k] = (uint32_t)(jw[k] >> 32);
+    }
+    const uint32_t lo = metropolis32((uint32_t)s, nlo, jlo, R0, R1, R2);
+    const uint32_t hi = metropolis32((uint32_t)(s >> 32), nhi, jhi, R0, R1, R2);
+    s = (uint64_t)lo | ((uint64_t)hi << 32);
+  } else {
+    const uint64_t b0 = s ^ nv[0] ^ jw[0];
+    const uint64_t b1 = s ^ nv[1] ^ jw[1];
+    const uint64_t b2 = s ^ nv[2] ^ jw[2];
+    const uint64_t b3 = s ^ nv[3] ^ jw[3];
+    const uint64_t b4 = s ^ nv[4] ^ jw[4];
+    const uint64_t b5 = s ^ nv[5] ^ jw[5];
+    const uint64_t s1 = b0 ^ b1 ^ b2, c1 = maj3(b0, b1, b2);
+    const uint64_t s2 = b3 ^ b4 ^ b5, c2 = maj3(b3, b4, b5);
+    const uint64_t u0 = s1 ^ s2, c3 = s1 & s2;
+    const uint64_t u1 = c1 ^ c2 ^ c3, u2 = maj3(c1, c2, c3);
+    int k0 = 0, k1 = 0;
+#pragma unroll
+    for (int u = 0; u < 7; ++u) {
+      k0 += ((uint32_t)u < tr.nfin[0]) & (rr >= tr.t[0][u]);
+      k1 += ((uint32_t)u < tr.nfin[1]) & (rr >= tr.t[1][u]);
+    }
+    const uint64_t f0 = count_ge(u2, u1, u0, k0);
+    const uint64_t f1 = count_ge(u2, u1, u0, k1);
+    s ^= (s & f1) | (~s & f0);
+  }
+}
 
-  const LatDesc d = descs[blockIdx.x];
+// bit-sliced unsatisfied-bond count (u2 u1 u0) of one site
+__device__ __forceinline__ void bond_count(uint64_t s, const uint64_t (&nv)[6],
+                                           const uint64_t (&jw)[6], uint64_t &u0,
+                                           uint64_t &u1, uint64_t &u2) {
+  const uint64_t b0 = s ^ nv[0] ^ jw[0];
+  const uint64_t b1 = s ^ nv[1] ^ jw[1];
+  const uint64_t b2 = s ^ nv[2] ^ jw[2];
+  const uint64_t b3 = s ^ nv[3] ^ jw[3];
+  const uint64_t b4 = s ^ nv[4] ^ jw[4];
+  const uint64_t b5 = s ^ nv[5] ^ jw[5];
+  const uint64_t s1 = b0 ^ b1 ^ b2, c1 = maj3(b0, b1, b2);
+  const uint64_t s2 = b3 ^ b4 ^ b5, c2 = maj3(b3, b4, b5);
+  const uint64_t c3 = s1 & s2;
+  u0 = s1 ^ s2;
+  u1 = c1 ^ c2 ^ c3;
+  u2 = maj3(c1, c2, c3);
+}
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+template <int L, bool FIELD>
+__global__ __launch_bounds__(Geo<L>::NT) void k_block(
+    uint64_t *__restrict__ spins, const uint64_t *__restrict__ Jg,
+    const LatDesc *__restrict__ descs, const TaskDesc *__restrict__ tasks,
+    const PairDesc *__restrict__ pairs, const ThrRec *__restrict__ thr,
+    const JobState *__restrict__ jobs, int do_pt, int nsweeps, int want_energy,
+    int32_t *__restrict__ energy, unsigned long long *__restrict__ counters) {
+  using G = Geo<L>;
+  // colour 0 (black) and colour 1 (white) halves of the two lattices as
+  // separate arrays: a phase writes one and reads the other, so the compiler
+  // may reorder the loads and stores of a phase freely
+  __shared__ uint64_t latB[2][G::NH], latW[2][G::NH];
+  __shared__ uint64_t red[G::NW * 24];
+  __shared__ unsigned long long smask;
+
+  const TaskDesc tk = tasks[blockIdx.x];
+  const bool two = tk.lat[1] != kNone;
+  const LatDesc d0 = descs[tk.lat[0]];
+  const LatDesc d1 = descs[two ? tk.lat[1] : tk.lat[0]];
   const int tid = threadIdx.x;
   const bool active = tid < G::Q;
-  uint64_t *gl = spins + d.spin_off;
-  for (int i = tid; i < G::N; i += G::NT) lat[i] = gl[i];
+  {
+    const uint64_t *g0 = spins + d0.spin_off, *g1 = spins + d1.spin_off;
+    for (int i = tid; i < G::NH; i += G::NT) {
+      latB[0][i] = g0[i];
+      latW[0][i] = g0[G::NH + i];
+    }
+    if (two)
+      for (int i = tid; i < G::NH; i += G::NT) {
+        latB[1][i] = g1[i];
+        latW[1][i] = g1[G::NH + i];
+      }
+  }
 
-  const uint64_t *Jx = Jg + d.j_off, *Jy = Jx + G::N, *Jz = Jy + G::N;
-  int nb[2][2][6];
+  // neighbour indices within the opposite colour's half (y, z: same for both
+  // colours; x: per colour) and the 6 couplings of each owned site
+  const uint64_t *Jx = Jg + d0.j_off, *Jy = Jx + G::N, *Jz = Jy + G::N;
+  int yz[2][4], xi[2][2][2];
   uint64_t jw[2][2][6];
   if (active) {
 #pragma unroll
@@ -217,16 +327,16 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_sweep_lds(
       const int xh = h % G::LH, y = (h / G::LH) % L, z = h / (G::LH * L);
       const int yp = (y + 1) % L, ym = (y + L - 1) % L;
       const int zp = (z + 1) % L, zm = (z + L - 1) % L;
+      yz[q][0] = xh + G::LH * (yp + L * z);
+      yz[q][1] = xh + G::LH * (ym + L * z);
+      yz[q][2] = xh + G::LH * (y + L * zp);
+      yz[q][3] = xh + G::LH * (y + L * zm);
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const int x = 2 * xh + ((y + z + c) & 1);
         const int xp = (x + 1) % L, xm = (x + L - 1) % L;
-        nb[q][c][0] = cb_index<L>(xp, y, z);
-        nb[q][c][1] = cb_index<L>(xm, y, z);
-        nb[q][c][2] = cb_index<L>(x, yp, z);
-        nb[q][c][3] = cb_index<L>(x, ym, z);
-        nb[q][c][4] = cb_index<L>(x, y, zp);
-        nb[q][c][5] = cb_index<L>(x, y, zm);
+        xi[q][c][0] = (xp >> 1) + G::LH * (y + L * z);
+        xi[q][c][1] = (xm >> 1) + G::LH * (y + L * z);
         const int s = x + L * (y + L * z);
         jw[q][c][0] = Jx[s];
         jw[q][c][1] = Jx[xm + L * (y + L * z)];
@@ -237,76 +347,77 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_sweep_lds(
       }
     }
   }
-  ThrRec tr{};
+  ThrRec tr0{}, tr1{};
   uint32_t n0 = 0;
-  if (nsweeps > 0) {  // an energy-only launch passes no tables
-    tr = thr[d.thr_idx];
-    n0 = jobs[d.job].sweep;
+  if (nsweeps > 0) {
+    tr0 = thr[d0.thr_idx];
+    tr1 = thr[d1.thr_idx];
+    n0 = jobs[d0.job].sweep;
   }
-  const uint32_t sw3 = (kTagMC << 28) | (d.rt & 0x0FFFFFFFu);
   __syncthreads();
-  // the thread's own four sites stay in registers between phases
-  uint64_t own[2][2];
-#pragma unroll
-  for (int c = 0; c < 2; ++c)
-#pragma unroll
-    for (int q = 0; q < 2; ++q) own[c][q] = active ? lat[c * G::NH + tid + q * G::Q] : 0;
 
+  // ---- PT exchange of the pair (t, t+1) at step b = sweep / pt_every
+  if (do_pt && two && tk.pair != kNone) {
+    const PairDesc p = pairs[tk.pair];
+    const JobState js = jobs[p.job];
+    if (tid < 64) {
+      const uint32_t b = js.sweep / js.pt_every;
+      const int l = tid;
+      const uint32_t sw3 = (kTagPT << 28) | (p.rt & 0x0FFFFFFFu);
+      const u32x4 r = philox_dev((uint32_t)(l >> 2), b, p.g, sw3, p.key0, p.key1);
+      const uint32_t rr = r.v[l & 3];
+      const int32_t *Ea = energy + (size_t)p.e_a * 128, *Eb = energy + (size_t)p.e_b * 128;
+      const bool acc = pt_accept(p.beta_a, p.beta_b, Ea[l], Eb[l], Ea[64 + l], Eb[64 + l], p.H, rr);
+      const unsigned long long m = __ballot(acc);
+      if (l == 0) {
+        smask = m;
+        atomicAdd(&counters[p.cnt_idx], (unsigned long long)__popcll(m));
+      }
+    }
+    __syncthreads();
+    const uint64_t m = smask;
+    if (m)
+      for (int i = tid; i < G::NH; i += G::NT) {
+        uint64_t a = latB[0][i], bb = latB[1][i], dd = (a ^ bb) & m;
+        latB[0][i] = a ^ dd;
+        latB[1][i] = bb ^ dd;
+        a = latW[0][i]; bb = latW[1][i]; dd = (a ^ bb) & m;
+        latW[0][i] = a ^ dd;
+        latW[1][i] = bb ^ dd;
+      }
+    __syncthreads();
+  }
+
+  const uint32_t sw0 = (kTagMC << 28) | (d0.rt & 0x0FFFFFFFu);
+  const uint32_t sw1 = (kTagMC << 28) | (d1.rt & 0x0FFFFFFFu);
   for (int it = 0; it < nsweeps; ++it) {
-    const u32x4 r = philox_dev((uint32_t)tid, n0 + (uint32_t)it, d.g, sw3, d.key0, d.key1);
+    const uint32_t n = n0 + (uint32_t)it;
+    const u32x4 r0 = philox_dev((uint32_t)tid, n, d0.g, sw0, d0.key0, d0.key1);
+    u32x4 r1 = r0;
+    if (two) r1 = philox_dev((uint32_t)tid, n, d1.g, sw1, d1.key0, d1.key1);
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       if (active) {
-        uint64_t nv[2][6];
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+        for (int a = 0; a < 2; ++a) {
+          if (a == 1 && !two) break;
+          uint64_t *mine = c == 0 ? latB[a] : latW[a];
+          const uint64_t *opp = c == 0 ? latW[a] : latB[a];
+          const ThrRec &tr = a == 0 ? tr0 : tr1;
+          const u32x4 &r = a == 0 ? r0 : r1;
 #pragma unroll
-          for (int k = 0; k < 6; ++k) nv[q][k] = lat[nb[q][c][k]];
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const int self = c * G::NH + tid + q * G::Q;
-          const uint64_t s = own[c][q];
-          const uint32_t rr = r.v[2 * c + q];
-          uint64_t snew;
-          if (!FIELD) {
-            // u >= 3 always flips; u = 2, 1, 0 flip iff rr < T(u)
-            const uint32_t R0 = rr < tr.t[0][0] ? ~0u : 0u;
-            const uint32_t R1 = rr < tr.t[0][1] ? ~0u : 0u;
-            const uint32_t R2 = rr < tr.t[0][2] ? ~0u : 0u;
-            uint32_t nlo[6], nhi[6], jlo[6], jhi[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-              nlo[k] = (uint32_t)nv[q][k];
-              nhi[k] = (uint32_t)(nv[q][k] >> 32);
-              jlo[k] = (uint32_t)jw[q][c][k];
-              jhi[k] = (uint32_t)(jw[q][c][k] >> 32);
-            }
-            const uint32_t lo = metropolis32((uint32_t)s, nlo, jlo, R0, R1, R2);
-            const uint32_t hi = metropolis32((uint32_t)(s >> 32), nhi, jhi, R0, R1, R2);
-            snew = (uint64_t)lo | ((uint64_t)hi << 32);
-          } else {
-            const uint64_t b0 = s ^ nv[q][0] ^ jw[q][c][0];
-            const uint64_t b1 = s ^ nv[q][1] ^ jw[q][c][1];
-            const uint64_t b2 = s ^ nv[q][2] ^ jw[q][c][2];
-            const uint64_t b3 = s ^ nv[q][3] ^ jw[q][c][3];
-            const uint64_t b4 = s ^ nv[q][4] ^ jw[q][c][4];
-            const uint64_t b5 = s ^ nv[q][5] ^ jw[q][c][5];
-            const uint64_t s1 = b0 ^ b1 ^ b2, c1 = maj3(b0, b1, b2);
-            const uint64_t s2 = b3 ^ b4 ^ b5, c2 = maj3(b3, b4, b5);
-            const uint64_t u0 = s1 ^ s2, c3 = s1 & s2;
-            const uint64_t u1 = c1 ^ c2 ^ c3, u2 = maj3(c1, c2, c3);
-            int k0 = 0, k1 = 0;
-#pragma unroll
-            for (int u = 0; u < 7; ++u) {
-              k0 += ((uint32_t)u < tr.nfin[0]) & (rr >= tr.t[0][u]);
-              k1 += ((uint32_t)u < tr.nfin[1]) & (rr >= tr.t[1][u]);
-            }
-            const uint64_t f0 = count_ge(u2, u1, u0, k0);
-            const uint64_t f1 = count_ge(u2, u1, u0, k1);
-            snew = s ^ ((s & f1) | (~s & f0));
+          for (int q = 0; q < 2; ++q) {
+            uint64_t nv[6];
+            nv[0] = opp[xi[q][c][0]];
+            nv[1] = opp[xi[q][c][1]];
+            nv[2] = opp[yz[q][0]];
+            nv[3] = opp[yz[q][1]];
+            nv[4] = opp[yz[q][2]];
+            nv[5] = opp[yz[q][3]];
+            uint64_t s = mine[tid + q * G::Q];
+            update_site<L, FIELD>(s, nv, jw[q][c], r.v[2 * c + q], tr);
+            mine[tid + q * G::Q] = s;
           }
-          own[c][q] = snew;
-          lat[self] = snew;
         }
       }
       __syncthreads();
@@ -314,38 +425,59 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_sweep_lds(
   }
 
   if (want_energy) {
-    // every bond touches exactly one black site (colour 0)
-    uint64_t AU[4] = {0, 0, 0, 0};
-    uint64_t AK[3] = {0, 0, 0};
+    // every bond touches exactly one black site (colour 0); the bit-sliced
+    // counts of both lattices are formed first, so the couplings are dead
+    // before the reductions
+    uint64_t AU[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    uint64_t AK[2][3] = {{0, 0, 0}, {0, 0, 0}};
     if (active) {
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int self = tid + q * G::Q;
-        const uint64_t s = lat[self];
-        const uint64_t b0 = s ^ lat[nb[q][0][0]] ^ jw[q][0][0];
-        const uint64_t b1 = s ^ lat[nb[q][0][1]] ^ jw[q][0][1];
-        const uint64_t b2 = s ^ lat[nb[q][0][2]] ^ jw[q][0][2];
-        const uint64_t b3 = s ^ lat[nb[q][0][3]] ^ jw[q][0][3];
-        const uint64_t b4 = s ^ lat[nb[q][0][4]] ^ jw[q][0][4];
-        const uint64_t b5 = s ^ lat[nb[q][0][5]] ^ jw[q][0][5];
-        const uint64_t s1 = b0 ^ b1 ^ b2, c1 = maj3(b0, b1, b2);
-        const uint64_t s2 = b3 ^ b4 ^ b5, c2 = maj3(b3, b4, b5);
-        const uint64_t u0 = s1 ^ s2, c3 = s1 & s2;
-        const uint64_t u1 = c1 ^ c2 ^ c3, u2 = maj3(c1, c2, c3);
-        vc_add3(AU, u0, u1, u2);
-        vc_add1(AK, s);
-        vc_add1(AK, lat[G::NH + self]);
+      for (int a = 0; a < 2; ++a) {
+        if (a == 1 && !two) break;
+        const uint64_t *opp = latW[a];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const uint64_t s = latB[a][tid + q * G::Q];
+          uint64_t nv[6];
+          nv[0] = opp[xi[q][0][0]];
+          nv[1] = opp[xi[q][0][1]];
+          nv[2] = opp[yz[q][0]];
+          nv[3] = opp[yz[q][1]];
+          nv[4] = opp[yz[q][2]];
+          nv[5] = opp[yz[q][3]];
+          uint64_t u0, u1, u2;
+          bond_count(s, nv, jw[q][0], u0, u1, u2);
+          vc_add3(AU[a], u0, u1, u2);
+          vc_add1(AK[a], s);
+          vc_add1(AK[a], latW[a][tid + q * G::Q]);
+        }
       }
     }
-    const int32_t U = wg_sum_planes<4, G::NW>(AU, red);
-    const int32_t K = wg_sum_planes<3, G::NW>(AK, red);
-    if (tid < 64) {
-      energy[(size_t)d.e_idx * 128 + tid] = U;
-      energy[(size_t)d.e_idx * 128 + 64 + tid] = K;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      if (a == 1 && !two) break;
+      const LatDesc &d = a == 0 ? d0 : d1;
+      const int32_t U = wg_sum_planes<4, G::NW>(AU[a], red);
+      if (tid < 64) energy[(size_t)d.e_idx * 128 + tid] = U;
+      if (want_energy > 1) {
+        const int32_t K = wg_sum_planes<3, G::NW>(AK[a], red);
+        if (tid < 64) energy[(size_t)d.e_idx * 128 + 64 + tid] = K;
+      }
     }
   }
 
-  for (int i = tid; i < G::N; i += G::NT) gl[i] = lat[i];
+  {
+    uint64_t *g0 = spins + d0.spin_off, *g1 = spins + d1.spin_off;
+    for (int i = tid; i < G::NH; i += G::NT) {
+      g0[i] = latB[0][i];
+      g0[G::NH + i] = latW[0][i];
+    }
+    if (two)
+      for (int i = tid; i < G::NH; i += G::NT) {
+        g1[i] = latB[1][i];
+        g1[G::NH + i] = latW[1][i];
+      }
+  }
 }
 
 // --------------------------------------------------------------- PT kernel
@@ -565,31 +697,29 @@ bool lds_kernel_supported(int L) {
   }
 }
 
-hipError_t launch_sweep(int L, bool field, uint64_t *spins, const uint64_t *J,
-                        const LatDesc *descs, int ndesc, const ThrRec *thr,
-                        const JobState *jobs, int nsweeps, int want_energy,
-                        int32_t *energy, hipStream_t st) {
-  if (ndesc <= 0) return hipSuccess;
+hipError_t launch_block(int L, bool field, uint64_t *spins, const uint64_t *J,
+                        const LatDesc *descs, const TaskDesc *tasks, int ntasks,
+                        const PairDesc *pairs, const ThrRec *thr, const JobState *jobs,
+                        int do_pt, int nsweeps, int want_energy, int32_t *energy,
+                        unsigned long long *counters, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
   switch (L) {
-#define X(l)                                                                              \
-  case l:                                                                                 \
-    if (field)                                                                            \
-      hipLaunchKernelGGL((k_sweep_lds<l, true>), dim3(ndesc), dim3(Geo<l>::NT), 0, st,    \
-                         spins, J, descs, thr, jobs, nsweeps, want_energy, energy);       \
-    else                                                                                  \
-      hipLaunchKernelGGL((k_sweep_lds<l, false>), dim3(ndesc), dim3(Geo<l>::NT), 0, st,   \
-                         spins, J, descs, thr, jobs, nsweeps, want_energy, energy);       \
+#define X(l)                                                                             \
+  case l:                                                                                \
+    if (field)                                                                           \
+      hipLaunchKernelGGL((k_block<l, true>), dim3(ntasks), dim3(Geo<l>::NT), 0, st,      \
+                         spins, J, descs, tasks, pairs, thr, jobs, do_pt, nsweeps,       \
+                         want_energy, energy, counters);                                 \
+    else                                                                                 \
+      hipLaunchKernelGGL((k_block<l, false>), dim3(ntasks), dim3(Geo<l>::NT), 0, st,     \
+                         spins, J, descs, tasks, pairs, thr, jobs, do_pt, nsweeps,       \
+                         want_energy, energy, counters);                                 \
     return hipGetLastError();
     SG_LDS_SIZES(X)
 #undef X
     default:
       return hipErrorInvalidValue;
   }
-}
-
-hipError_t launch_energy(int L, uint64_t *spins, const uint64_t *J, const LatDesc *descs,
-                         int ndesc, int32_t *energy, hipStream_t st) {
-  return launch_sweep(L, false, spins, J, descs, ndesc, nullptr, nullptr, 0, 1, energy, st);
 }
 
 hipError_t launch_pt(uint64_t *spins, int32_t *energy, const PairDesc *pairs, int npairs,

@@ -82,6 +82,11 @@ struct sg_handle {
   ObsAcc *d_acc = nullptr;        // [group][temp][lane]
   double *d_cs = nullptr;         // cos, sin of 2 pi c / L
   int npairs = 0, nfield = 0;
+  // work-group tasks of the sweep kernel for PT steps of parity 0 and 1:
+  // pairs (t, t+1) with t of that parity, the other slots single; ordered by
+  // group (task_off[p][g] .. task_off[p][g+1])
+  TaskDesc *d_tasks[2] = {nullptr, nullptr};
+  std::vector<int> task_off[2];
 
   // records for the sink: the measurement kernel writes d_rec, an async copy
   // moves it to one of kRing pinned buffers, and the sink runs once the copy
@@ -143,18 +148,20 @@ struct sg_handle {
     pending.clear();
   }
 
-  hipGraphExec_t graph = nullptr;   // batched form: one PT interval
-  int graph_seg = -1;
-  bool graph_energy = false;
+  // batched form: one PT interval (sweep launch + counter advance) per
+  // (parity, length, energy) captured once as a hipGraph and replayed
+  struct GraphKey { int parity, seg, energy; hipGraphExec_t exec; };
+  std::vector<GraphKey> graphs;
 
   ~sg_handle() { release(); }
 
   void release() {
-    if (graph) hipGraphExecDestroy(graph);
-    graph = nullptr;
     hipFree(d_spins); hipFree(d_J); hipFree(d_thr); hipFree(d_energy); hipFree(d_rec);
     hipFree(d_lat); hipFree(d_pairs); hipFree(d_meas); hipFree(d_jobs); hipFree(d_cnt);
-    hipFree(d_acc); hipFree(d_cs);
+    hipFree(d_acc); hipFree(d_cs); hipFree(d_tasks[0]); hipFree(d_tasks[1]);
+    d_tasks[0] = d_tasks[1] = nullptr;
+    for (auto &g : graphs) hipGraphExecDestroy(g.exec);
+    graphs.clear();
     d_spins = d_J = nullptr; d_thr = nullptr; d_energy = d_rec = nullptr; d_lat = nullptr;
     d_pairs = nullptr; d_meas = nullptr; d_jobs = nullptr; d_cnt = nullptr;
     d_acc = nullptr; d_cs = nullptr;
@@ -258,6 +265,41 @@ struct sg_handle {
         }
     if (npairs)
       HIPCHK(hipMemcpy(d_pairs, pd.data(), sizeof(PairDesc) * npairs, hipMemcpyHostToDevice));
+
+    for (int par = 0; par < 2; ++par) {
+      std::vector<TaskDesc> tv;
+      task_off[par].assign(G + 1, 0);
+      for (int g = 0; g < G; ++g) {
+        task_off[par][g] = (int)tv.size();
+        for (int rho = 0; rho < R; ++rho)
+          for (int t = 0; t < T;) {
+            TaskDesc k;
+            k.pad = 0;
+            k.lat[0] = (uint32_t)lat_index(g, rho, t);
+            if ((t & 1) == par && t + 1 < T) {
+              k.lat[1] = (uint32_t)lat_index(g, rho, t + 1);
+              k.pair = (uint32_t)(((size_t)g * R + rho) * (T - 1) + t);
+              t += 2;
+            } else {
+              k.lat[1] = 0xFFFFFFFFu;
+              k.pair = 0xFFFFFFFFu;
+              t += 1;
+            }
+            tv.push_back(k);
+          }
+        // within a group, pairs (twice the work) first, then the single
+        // lattices, so that the dispatcher's greedy fill ends evenly
+        std::stable_partition(tv.begin() + task_off[par][g], tv.end(),
+                              [](const TaskDesc &k) { return k.lat[1] != 0xFFFFFFFFu; });
+      }
+      if (form == SG_FORM_BATCHED)  // one launch: all pairs of all groups first
+        std::stable_partition(tv.begin(), tv.end(),
+                              [](const TaskDesc &k) { return k.lat[1] != 0xFFFFFFFFu; });
+      task_off[par][G] = (int)tv.size();
+      dev_alloc(&d_tasks[par], tv.size());
+      HIPCHK(hipMemcpy(d_tasks[par], tv.data(), sizeof(TaskDesc) * tv.size(),
+                       hipMemcpyHostToDevice));
+    }
 
     std::vector<MeasDesc> md((size_t)G * T);
     for (int g = 0; g < G; ++g)
@@ -372,59 +414,47 @@ struct sg_handle {
   }
 
   // ------------------------------------------------------------ run loop
-  void launch_interval(int seg, bool want_energy) {
-    if (form == SG_FORM_PLAIN) {
-      // one group per launch (the per-sample-group drop-in form)
-      const int per_g = R * T;
-      const double fl = (double)seg * N * 64.0 * per_g;
-      for (int g = 0; g < G; ++g)
-        timed(0, fl, [&] {
-          HIPCHK(launch_sweep(L, field, d_spins, d_J, d_lat + (size_t)g * per_g, per_g, d_thr,
-                              d_jobs, seg, want_energy ? 1 : 0, d_energy, st));
-        });
-    } else {
-      const double fl = (double)seg * N * 64.0 * n_lat();
+  int energy_mode() const { return field ? 2 : 1; }   // K enters PT only when H != 0
+
+  // One sweep-kernel launch over the tasks of parity `par` (all groups in the
+  // batched form, one launch per group in the plain form), then the device
+  // sweep counter advance.  do_pt: the launch starts at a PT step.
+  void launch_interval(int par, int do_pt, int seg, int want_energy) {
+    const bool per_group = form == SG_FORM_PLAIN;
+    for (int g = 0; g < (per_group ? G : 1); ++g) {
+      const int a = per_group ? task_off[par][g] : 0;
+      const int b = per_group ? task_off[par][g + 1] : task_off[par][G];
+      const double fl = (double)seg * N * 64.0 * R * T * (per_group ? 1 : G);
       timed(0, fl, [&] {
-        HIPCHK(launch_sweep(L, field, d_spins, d_J, d_lat, (int)n_lat(), d_thr, d_jobs, seg,
-                            want_energy ? 1 : 0, d_energy, st));
+        HIPCHK(launch_block(L, field, d_spins, d_J, d_lat, d_tasks[par] + a, b - a, d_pairs,
+                            d_thr, d_jobs, do_pt, seg, want_energy, d_energy, d_cnt, st));
       });
     }
-    HIPCHK(launch_advance(d_jobs, 1, seg, st));
+    if (seg > 0) HIPCHK(launch_advance(d_jobs, 1, seg, st));
   }
 
-  void launch_pt_step() {
-    if (form == SG_FORM_PLAIN) {
-      const int per_g = R * (T - 1);
-      for (int g = 0; g < G; ++g)
-        timed(1, 0.0, [&] {
-          HIPCHK(launch_pt(d_spins, d_energy, d_pairs + (size_t)g * per_g, per_g, d_jobs,
-                           d_cnt, st));
-        });
-    } else {
-      timed(1, 0.0, [&] {
-        HIPCHK(launch_pt(d_spins, d_energy, d_pairs, npairs, d_jobs, d_cnt, st));
-      });
-    }
+  void launch_energy_all(int mode) {
+    HIPCHK(launch_block(L, field, d_spins, d_J, d_lat, d_tasks[0], task_off[0][G], d_pairs,
+                        d_thr, d_jobs, 0, 0, mode, d_energy, d_cnt, st));
   }
 
-  // Batched form: the PT step plus a full interval of pt_every sweeps is one
-  // hipGraph, replayed while the device sweep counter advances.
-  void run_graph_interval(int seg, bool want_energy) {
-    if (!graph || graph_seg != seg || graph_energy != want_energy) {
-      if (graph) HIPCHK(hipGraphExecDestroy(graph));
-      graph = nullptr;
+  // Batched form: a full PT interval is one hipGraph per (parity, length,
+  // energy), replayed while the device sweep counter advances.
+  void run_graph_interval(int par, int seg, int want_energy) {
+    hipGraphExec_t exec = nullptr;
+    for (auto &k : graphs)
+      if (k.parity == par && k.seg == seg && k.energy == want_energy) exec = k.exec;
+    if (!exec) {
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      launch_pt_step();
-      launch_interval(seg, want_energy);
+      launch_interval(par, 1, seg, want_energy);
       HIPCHK(hipStreamEndCapture(st, &g));
-      hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+      hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
       hipGraphDestroy(g);
       HIPCHK(e);
-      graph_seg = seg;
-      graph_energy = want_energy;
+      graphs.push_back(GraphKey{par, seg, want_energy, exec});
     }
-    HIPCHK(hipGraphLaunch(graph, st));
+    HIPCHK(hipGraphLaunch(exec, st));
   }
 
   void run(int64_t n_sweeps, sg_obs_sink sink, void *user) {
@@ -442,18 +472,16 @@ struct sg_handle {
       const int64_t next = n + seg;
       const bool want_energy = pt && next % pt_every == 0;
       const bool pt_now = pt && n % pt_every == 0;
-      if (pt_now && !energy_valid)
-        HIPCHK(launch_energy(L, d_spins, d_J, d_lat, (int)n_lat(), d_energy, st));
+      if (pt_now && !energy_valid) launch_energy_all(energy_mode());
+      const int par = pt_now ? (int)((n / pt_every) & 1) : 0;
+      const int emode = want_energy ? energy_mode() : 0;
       if (form == SG_FORM_BATCHED && pt_now && seg == pt_every && !profiling) {
-        run_graph_interval((int)seg, want_energy);
+        run_graph_interval(par, (int)seg, emode);
       } else {
-        if (pt_now) launch_pt_step();
-        launch_interval((int)seg, want_energy);
+        launch_interval(par, pt_now ? 1 : 0, (int)seg, emode);
       }
-      if (pt_now) {
-        const int parity = (int)((n / pt_every) & 1);
-        for (int t = parity; t + 1 < T; t += 2) swap_att[t] += (uint64_t)G * R * 64;
-      }
+      if (pt_now)
+        for (int t = par; t + 1 < T; t += 2) swap_att[t] += (uint64_t)G * R * 64;
       sweep = next;
       energy_valid = want_energy;
       flips += (double)seg * N * 64.0 * G * R * T;
@@ -769,8 +797,8 @@ sg_status sg_set_couplings(sg_handle *h, const uint64_t *in) {
 sg_status sg_get_energies(sg_handle *h, int32_t *U, int32_t *K) {
   if (!U || !K) return SG_ERR_INVALID;
   return guarded(h, [&] {
-    if (!h->energy_valid)
-      HIPCHK(launch_energy(h->L, h->d_spins, h->d_J, h->d_lat, (int)h->n_lat(), h->d_energy, h->st));
+    // always a fresh pass: the slots of the last PT step hold K only at H != 0
+    h->launch_energy_all(2);
     h->energy_valid = true;
     std::vector<int32_t> e(h->n_lat() * 128);
     HIPCHK(hipStreamSynchronize(h->st));
