@@ -13,8 +13,9 @@ struct JobState {
   uint32_t pad[2];
 };
 
-// largest L whose lattice the LDS-resident sweep kernel holds
-bool lds_kernel_supported(int L);
+// a sweep kernel exists for every even L in [2, 48]: k_block (lattice in
+// LDS, couplings in VGPRs) up to L = 16, k_big above
+bool sweep_kernel_supported(int L);
 
 // One work-group per task (one lattice, or the lattices of slots t and t+1
 // of one group and replica): with do_pt, the task's PT pair is decided and

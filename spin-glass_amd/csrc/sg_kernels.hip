@@ -480,6 +480,219 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   }
 }
 
+// ------------------------------------------------------- large-lattice kernel
+// L >= 18: a lattice no longer fits the VGPR budget of one Philox quad per
+// thread, so each thread walks several quads (j = tid, tid + NT, ...), reads
+// the couplings of each site from L2, and draws the quad's Philox block in
+// each phase.  With IN_LDS the task's lattices are copied into LDS for the
+// launch (L = 18, 20: two lattices fit in 160 KB); otherwise they are updated
+// in place in global memory (L2/MALL-resident), which is coherent within the
+// work-group because all its waves share one CU.  Same task / PT / energy
+// semantics as k_block.
+template <int L>
+struct BigGeo {
+  static constexpr int N = L * L * L;
+  static constexpr int NH = N / 2;
+  static constexpr int Q = N / 4;
+  static constexpr int LH = L / 2;
+  static constexpr int NT = 1024;
+  static constexpr int NW = NT / 64;
+  static constexpr int QPT = (Q + NT - 1) / NT;   // quads per thread
+  // bits of a per-thread count of up to 12 * QPT (U) or 4 * QPT (K)
+  static constexpr int PU = QPT <= 1 ? 4 : QPT <= 2 ? 5 : QPT <= 5 ? 6 : QPT <= 10 ? 7 : QPT <= 21 ? 8 : 9;
+  static constexpr int PK = QPT <= 1 ? 3 : QPT <= 3 ? 4 : QPT <= 7 ? 5 : QPT <= 15 ? 6 : 7;
+};
+
+template <int L>
+__device__ __forceinline__ void site_geometry(int h, int c, int (&nb)[6], int &s_nat,
+                                              int (&jn)[6]) {
+  using G = BigGeo<L>;
+  const int xh = h % G::LH, y = (h / G::LH) % L, z = h / (G::LH * L);
+  const int x = 2 * xh + ((y + z + c) & 1);
+  const int xp = (x + 1) % L, xm = (x + L - 1) % L;
+  const int yp = (y + 1) % L, ym = (y + L - 1) % L;
+  const int zp = (z + 1) % L, zm = (z + L - 1) % L;
+  // indices into the opposite colour's half
+  nb[0] = (xp >> 1) + G::LH * (y + L * z);
+  nb[1] = (xm >> 1) + G::LH * (y + L * z);
+  nb[2] = xh + G::LH * (yp + L * z);
+  nb[3] = xh + G::LH * (ym + L * z);
+  nb[4] = xh + G::LH * (y + L * zp);
+  nb[5] = xh + G::LH * (y + L * zm);
+  s_nat = x + L * (y + L * z);
+  // natural indices of the six bonds' owners: +x, -x, +y, -y, +z, -z
+  jn[0] = s_nat;
+  jn[1] = xm + L * (y + L * z);
+  jn[2] = s_nat + G::N;
+  jn[3] = x + L * (ym + L * z) + G::N;
+  jn[4] = s_nat + 2 * G::N;
+  jn[5] = x + L * (y + L * zm) + 2 * G::N;
+}
+
+template <int L, bool FIELD, bool IN_LDS>
+__global__ __launch_bounds__(1024) void k_big(
+    uint64_t *__restrict__ spins, const uint64_t *__restrict__ Jg,
+    const LatDesc *__restrict__ descs, const TaskDesc *__restrict__ tasks,
+    const PairDesc *__restrict__ pairs, const ThrRec *__restrict__ thr,
+    const JobState *__restrict__ jobs, int do_pt, int nsweeps, int want_energy,
+    int32_t *__restrict__ energy, unsigned long long *__restrict__ counters) {
+  using G = BigGeo<L>;
+  constexpr int LDS_WORDS = IN_LDS ? 2 * G::N : 1;
+  __shared__ uint64_t slat[LDS_WORDS];
+  __shared__ uint64_t red[G::NW * 24];
+  __shared__ unsigned long long smask;
+
+  const TaskDesc tk = tasks[blockIdx.x];
+  const bool two = tk.lat[1] != kNone;
+  const LatDesc d0 = descs[tk.lat[0]];
+  const LatDesc d1 = descs[two ? tk.lat[1] : tk.lat[0]];
+  const int tid = threadIdx.x;
+  const int nl = two ? 2 : 1;
+  uint64_t *gl[2] = {spins + d0.spin_off, spins + d1.spin_off};
+  uint64_t *lat[2];
+  if (IN_LDS) {
+    lat[0] = slat;
+    lat[1] = slat + G::N;
+    for (int a = 0; a < nl; ++a)
+      for (int i = tid; i < G::N; i += G::NT) lat[a][i] = gl[a][i];
+  } else {
+    lat[0] = gl[0];
+    lat[1] = gl[1];
+  }
+  const uint64_t *J = Jg + d0.j_off;
+  ThrRec tr0{}, tr1{};
+  uint32_t n0 = 0;
+  if (nsweeps > 0) {
+    tr0 = thr[d0.thr_idx];
+    tr1 = thr[d1.thr_idx];
+    n0 = jobs[d0.job].sweep;
+  }
+  __syncthreads();
+
+  if (do_pt && two && tk.pair != kNone) {
+    const PairDesc p = pairs[tk.pair];
+    const JobState js = jobs[p.job];
+    if (tid < 64) {
+      const uint32_t b = js.sweep / js.pt_every;
+      const int l = tid;
+      const uint32_t sw3 = (kTagPT << 28) | (p.rt & 0x0FFFFFFFu);
+      const u32x4 r = philox_dev((uint32_t)(l >> 2), b, p.g, sw3, p.key0, p.key1);
+      const uint32_t rr = r.v[l & 3];
+      const int32_t *Ea = energy + (size_t)p.e_a * 128, *Eb = energy + (size_t)p.e_b * 128;
+      const bool acc = pt_accept(p.beta_a, p.beta_b, Ea[l], Eb[l], Ea[64 + l], Eb[64 + l], p.H, rr);
+      const unsigned long long m = __ballot(acc);
+      if (l == 0) {
+        smask = m;
+        atomicAdd(&counters[p.cnt_idx], (unsigned long long)__popcll(m));
+      }
+    }
+    __syncthreads();
+    const uint64_t m = smask;
+    if (m)
+      for (int i = tid; i < G::N; i += G::NT) {
+        const uint64_t a = lat[0][i], bb = lat[1][i], dd = (a ^ bb) & m;
+        lat[0][i] = a ^ dd;
+        lat[1][i] = bb ^ dd;
+      }
+    __syncthreads();
+  }
+
+  const uint32_t sw[2] = {(kTagMC << 28) | (d0.rt & 0x0FFFFFFFu),
+                          (kTagMC << 28) | (d1.rt & 0x0FFFFFFFu)};
+  for (int it = 0; it < nsweeps; ++it) {
+    const uint32_t n = n0 + (uint32_t)it;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll 1
+      for (int jq0 = tid; jq0 < G::Q; jq0 += G::NT) {
+        // opaque to the optimiser: when NT is a whole number of planes (L = 32)
+        // it would otherwise hoist ~20 per-thread 64-bit neighbour addresses
+        // out of the sweep loop and spill them
+        int jq = jq0;
+        asm volatile("" : "+v"(jq));
+        const u32x4 r0 = philox_dev((uint32_t)jq, n, d0.g, sw[0], d0.key0, d0.key1);
+        u32x4 r1 = r0;
+        if (two) r1 = philox_dev((uint32_t)jq, n, d1.g, sw[1], d1.key0, d1.key1);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int h = jq + q * G::Q;
+          int nb[6], s_nat, jn[6];
+          site_geometry<L>(h, c, nb, s_nat, jn);
+          uint64_t jw[6];
+#pragma unroll
+          for (int k = 0; k < 6; ++k) jw[k] = J[jn[k]];
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            if (a == 1 && !two) break;
+            const uint64_t *opp = lat[a] + (1 - c) * G::NH;
+            uint64_t nv[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) nv[k] = opp[nb[k]];
+            uint64_t s = lat[a][c * G::NH + h];
+            update_site<L, FIELD>(s, nv, jw, (a == 0 ? r0 : r1).v[2 * c + q], a == 0 ? tr0 : tr1);
+            lat[a][c * G::NH + h] = s;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  if (want_energy) {
+    // one lattice at a time keeps the live bit planes small
+#pragma unroll 1
+    for (int a = 0; a < 2; ++a) {
+      if (a == 1 && !two) break;
+      const uint64_t *la = a == 0 ? lat[0] : lat[1];
+      uint64_t AU[G::PU];
+#pragma unroll
+      for (int p = 0; p < G::PU; ++p) AU[p] = 0;
+#pragma unroll 1
+      for (int jq = tid; jq < G::Q; jq += G::NT) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int h = jq + q * G::Q;
+          int nb[6], s_nat, jn[6];
+          site_geometry<L>(h, 0, nb, s_nat, jn);
+          uint64_t jw[6], nv[6];
+#pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            jw[k] = J[jn[k]];
+            nv[k] = la[G::NH + nb[k]];
+          }
+          uint64_t u0, u1, u2;
+          bond_count(la[h], nv, jw, u0, u1, u2);
+          vc_add3(AU, u0, u1, u2);
+        }
+      }
+      const LatDesc &d = a == 0 ? d0 : d1;
+      const int32_t U = wg_sum_planes<G::PU, G::NW>(AU, red);
+      if (tid < 64) energy[(size_t)d.e_idx * 128 + tid] = U;
+      if (want_energy > 1) {
+        uint64_t AK[G::PK];
+#pragma unroll
+        for (int p = 0; p < G::PK; ++p) AK[p] = 0;
+#pragma unroll 1
+        for (int jq = tid; jq < G::Q; jq += G::NT) {
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const int h = jq + q * G::Q;
+            vc_add1(AK, la[h]);
+            vc_add1(AK, la[G::NH + h]);
+          }
+        }
+        const int32_t K = wg_sum_planes<G::PK, G::NW>(AK, red);
+        if (tid < 64) energy[(size_t)d.e_idx * 128 + 64 + tid] = K;
+      }
+    }
+  }
+
+  if (IN_LDS) {
+    for (int a = 0; a < nl; ++a)
+      for (int i = tid; i < G::N; i += G::NT) gl[a][i] = lat[a][i];
+  }
+}
+
 // --------------------------------------------------------------- PT kernel
 // One work-group per adjacent pair.  Wave 0 decides the 64 lanes (one
 // Philox word each), the ballot is the exchange mask, and the whole group
@@ -688,10 +901,10 @@ __global__ void k_advance(JobState *jobs, int njobs, int nsweeps) {
 #define SG_ALL_SIZES(X) SG_LDS_SIZES(X) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) \
   X(34) X(36) X(38) X(40) X(42) X(44) X(46) X(48)
 
-bool lds_kernel_supported(int L) {
+bool sweep_kernel_supported(int L) {
   switch (L) {
 #define X(l) case l: return true;
-    SG_LDS_SIZES(X)
+    SG_ALL_SIZES(X)
 #undef X
     default: return false;
   }
@@ -716,6 +929,21 @@ hipError_t launch_block(int L, bool field, uint64_t *spins, const uint64_t *J,
                          want_energy, energy, counters);                                 \
     return hipGetLastError();
     SG_LDS_SIZES(X)
+#undef X
+#define X(l, in_lds)                                                                     \
+  case l:                                                                                \
+    if (field)                                                                           \
+      hipLaunchKernelGGL((k_big<l, true, in_lds>), dim3(ntasks), dim3(1024), 0, st,      \
+                         spins, J, descs, tasks, pairs, thr, jobs, do_pt, nsweeps,       \
+                         want_energy, energy, counters);                                 \
+    else                                                                                 \
+      hipLaunchKernelGGL((k_big<l, false, in_lds>), dim3(ntasks), dim3(1024), 0, st,     \
+                         spins, J, descs, tasks, pairs, thr, jobs, do_pt, nsweeps,       \
+                         want_energy, energy, counters);                                 \
+    return hipGetLastError();
+    X(18, true) X(20, true) X(22, false) X(24, false) X(26, false) X(28, false)
+    X(30, false) X(32, false) X(34, false) X(36, false) X(38, false) X(40, false)
+    X(42, false) X(44, false) X(46, false) X(48, false)
 #undef X
     default:
       return hipErrorInvalidValue;

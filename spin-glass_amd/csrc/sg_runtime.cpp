@@ -703,7 +703,7 @@ sg_status sg_create(const sg_params *p, sg_handle **out) {
   auto bad = [](const char *m) { g_create_error = m; return SG_ERR_INVALID; };
   if (!p) return bad("params is NULL");
   if (p->L < 2 || p->L > kMaxL || (p->L & 1)) return bad("L must be even and in [2, 48]");
-  if (!lds_kernel_supported(p->L)) return bad("this build sweeps L <= 16 only");
+  if (!sweep_kernel_supported(p->L)) return bad("no sweep kernel for this L");
   if (p->n_samples < 1) return bad("n_samples must be >= 1");
   if (p->n_temps < 1 || p->n_temps > 4096) return bad("n_temps must be in [1, 4096]");
   if (p->n_replicas < 1 || p->n_replicas > 8) return bad("n_replicas must be in [1, 8]");

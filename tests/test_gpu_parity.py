@@ -64,6 +64,34 @@ def test_sweeps_pt_measurements_match_oracle(L):
     np.testing.assert_allclose(sim.results(), ref, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("L", [18, 20, 22, 24, 32, 48])
+def test_large_lattice_sweeps_match_oracle(L):
+    """L > 16 runs the large-lattice kernel: LDS-resident for L = 18, 20,
+    updated in place in global memory above; 80 samples = one full and one
+    partial lane group; 4 temperatures give pair-only and mixed task lists."""
+    betas = geometric_betas(0.3, 2.0, 4)
+    sim, orc = _pair(L, 80, 2, betas, pt_every=2, meas_every=3)
+    sim.run(4)
+    orc.run(4)
+    _assert_same(sim, orc)
+    sim.run(3)
+    orc.run(3)
+    _assert_same(sim, orc)
+    assert len(sim.records) == orc.n_meas() == 2
+    for i, (_, rec) in enumerate(sim.records):
+        np.testing.assert_array_equal(rec, orc.meas(i))
+
+
+@pytest.mark.parametrize("L", [20, 26])
+def test_large_lattice_field_matches_oracle(L):
+    betas = geometric_betas(0.3, 2.0, 3)
+    sim, orc = _pair(L, 64, 2, betas, H=0.5, pt_every=1, meas_every=3)
+    sim.run(5)
+    orc.run(5)
+    _assert_same(sim, orc)
+    np.testing.assert_array_equal(sim.records[0][1], orc.meas(0))
+
+
 @pytest.mark.parametrize("H", [0.0625, 0.5, 2.5])
 def test_field_matches_oracle(H):
     betas = geometric_betas(0.3, 2.0, 5)
