@@ -144,6 +144,70 @@ sg_status sg_set_profiling(sg_handle *h, int32_t on);
 sg_status sg_get_kernel_stats(sg_handle *h, int32_t kind, int64_t *launches,
                               double *total_ms, double *flips);
 
+/* --- batch and serving engine -------------------------------------------
+ * No counterpart in the reference, which runs one fixed-size simulation per
+ * process (Program/src/main.cu:8-84).  An engine holds many jobs (each a
+ * simulation as above, of any L, ladder, seeds and sweep count) on one
+ * device, their state resident in a device arena between steps.  Jobs are
+ * admitted in submission order while their footprint fits the arena; each
+ * step runs slice_pt_steps PT intervals of every running job, all jobs of one
+ * (L, field) class in one sweep-kernel launch per interval (SG_FORM_BATCHED)
+ * or one launch per job (SG_FORM_PLAIN).  A job's results are bit-identical
+ * to those of the same parameters run alone through sg_run: every random
+ * number is keyed by the job's own seeds and counters.  One host thread per
+ * engine. */
+typedef struct sg_batch sg_batch;
+
+typedef struct sg_batch_params {
+  int32_t device;
+  int32_t pt_every;        /* PT cadence of every job, sweeps (> 0) */
+  int32_t slice_pt_steps;  /* PT intervals per sg_step (>= 1) */
+  int32_t form;            /* SG_FORM_BATCHED or SG_FORM_PLAIN, as above */
+  int64_t arena_bytes;     /* device memory for job state */
+  int32_t max_inflight;    /* sg_step calls queued on the device ahead of the host (>= 1) */
+  int32_t pad;
+} sg_batch_params;
+
+enum { SG_JOB_QUEUED = 0, SG_JOB_RUNNING = 1, SG_JOB_DONE = 2 };
+
+void sg_batch_params_default(sg_batch_params *bp);
+sg_status sg_batch_create(const sg_batch_params *bp, sg_batch **out);
+sg_status sg_batch_destroy(sg_batch *b);
+/* message of the last failed call on this engine (or of the last failed
+ * sg_batch_create when b == NULL) */
+const char *sg_batch_last_error(const sg_batch *b);
+/* device bytes a job with these parameters takes from the arena */
+int64_t sg_job_footprint(const sg_params *p);
+/* queues a job of n_sweeps sweeps.  p->pt_every must be 0 or the engine's;
+ * n_sweeps and p->meas_every must be multiples of the engine's pt_every;
+ * p->form and p->device are ignored.  SG_ERR_NOMEM when the footprint
+ * exceeds the whole arena.  *id >= 0 names the job from then on. */
+sg_status sg_submit(sg_batch *b, const sg_params *p, int64_t n_sweeps, int64_t *id);
+/* one step: retire finished jobs, admit queued ones, run one slice of every
+ * running job.  Returns the counts after the step (either may be NULL). */
+sg_status sg_step(sg_batch *b, int32_t *n_running, int32_t *n_queued);
+/* steps until every submitted job is done */
+sg_status sg_drain(sg_batch *b);
+sg_status sg_poll(sg_batch *b, int64_t id, int32_t *state);
+/* [T][SG_NRES] results and [T-1] accepted swap counts (may be NULL) of a
+ * finished job; the job is released and its id becomes invalid */
+sg_status sg_result(sg_batch *b, int64_t id, double *res, uint64_t *swaps_accepted);
+/* host wall-clock seconds since the engine's creation: submitted (or the
+ * arrival time given to sg_serve), admitted, finished (seen done by the host) */
+sg_status sg_job_times(sg_batch *b, int64_t id, double *t_submit, double *t_admit,
+                       double *t_done);
+/* spin-flip attempts launched, steps run, arena bytes in use and capacity */
+sg_status sg_batch_stats(sg_batch *b, double *flips, int64_t *steps, int64_t *arena_in_use,
+                         int64_t *arena_capacity);
+
+/* Serving loop: jobs[i] arrives arrival_s[i] seconds after the call starts
+ * (non-decreasing); each is submitted when the host clock reaches its
+ * arrival, sg_step runs while any job is queued or running, and the loop
+ * returns once all are done.  ids[i] receives the job id (results stay
+ * available through sg_result); wall_s the loop's duration. */
+sg_status sg_serve(sg_batch *b, const sg_params *jobs, const int64_t *n_sweeps,
+                   const double *arrival_s, int32_t n_jobs, int64_t *ids, double *wall_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,7 @@ FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-ffp-contract=
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}"]
 
-LIB_SOURCES = ["sg_kernels.hip", "sg_runtime.cpp"]
+LIB_SOURCES = ["sg_kernels.hip", "sg_runtime.cpp", "sg_engine.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -115,8 +115,7 @@ struct LatDesc {
   uint32_t rt;         // (rho << 16) | t
   uint32_t thr_idx;    // ThrRec index
   uint32_t e_idx;      // energy slot: U at [e_idx*128 + l], K at [e_idx*128 + 64 + l]
-  uint32_t job;        // JobState index (sweep counter)
-  uint32_t pad;
+  uint32_t pad[2];
 };
 
 // One PT pair (t, t+1) of one (group, replica).
@@ -127,19 +126,42 @@ struct PairDesc {
   uint32_t e_a, e_b;      // energy slots
   uint32_t key0, key1;
   uint32_t g, rt;         // rt = (rho << 16) | t
-  uint32_t job;           // JobState index: step b = sweep / pt_every
+  uint32_t pad;
   uint32_t cnt_idx;       // accepted-swap counter index
   uint32_t n_words;       // N
   uint32_t parity;        // t & 1: the pair is active on steps with b & 1 == parity
 };
 
+// Per-job device state read by the kernels, so that a launch sequence can be
+// captured once in a hipGraph and replayed while the counters advance.
+struct JobState {
+  uint32_t sweep;     // sweeps completed; a PT step before sweep n has b = n / pt_every
+  uint32_t pt_every;  // PT cadence (0 = none)
+  uint32_t pad[2];
+};
+
+// The device buffers of one job (one simulation: one L, its groups,
+// replicas and temperatures).  Descriptor offsets and indices are relative
+// to these bases.
+struct JobPtrs {
+  uint64_t *spins;
+  const uint64_t *J;
+  const LatDesc *descs;
+  const PairDesc *pairs;
+  const ThrRec *thr;
+  JobState *js;
+  int32_t *energy;
+  unsigned long long *counters;
+};
+
 // One work-group task of the sweep kernel: lattice descriptor indices (the
-// second is 0xFFFFFFFF for a single lattice) and, when the two form a PT
-// pair (slots t and t+1), the PairDesc index.
+// second is 0xFFFFFFFF for a single lattice), when the two form a PT pair
+// (slots t and t+1) the PairDesc index, and the job's entry in the launch's
+// JobPtrs table.
 struct TaskDesc {
   uint32_t lat[2];
   uint32_t pair;
-  uint32_t pad;
+  uint32_t job;
 };
 
 // One (group, temperature) measured across its replicas.

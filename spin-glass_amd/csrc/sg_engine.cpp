@@ -1,0 +1,573 @@
+// sg_engine.cpp — batch and serving engine of libsg (sg_batch_* in
+// include/sg.h).
+//
+// Many simulations ("jobs") share one device.  Their state lives in one
+// device arena (sg_arena.h) from admission to retirement; a job is admitted,
+// in submission order, once its footprint fits.  Jobs of one (L, field)
+// class share a HIP stream, and in the batched form one sweep-kernel launch
+// per PT interval covers every task of every running job of the class
+// (TaskDesc.job indexes the class's JobPtrs table).  Classes run on separate
+// streams, so the device overlaps them.  Because each job's random numbers
+// are keyed by its own seeds and counters, the mix never changes a result.
+#include <chrono>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+
+#include "sg_handle.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Job {
+  int64_t id = -1;
+  sg_params p{};
+  std::vector<double> betas;
+  int64_t n_sweeps = 0;
+  int state = SG_JOB_QUEUED;
+  bool finishing = false;   // final copies queued, not yet seen complete
+  size_t footprint = 0;
+  int cls = -1;
+  std::unique_ptr<sg_handle> h;
+  ObsAcc *h_acc = nullptr;  // pinned copies of the final sums and counters
+  uint64_t *h_cnt = nullptr;
+  hipEvent_t done_ev = nullptr;
+  std::vector<double> res;
+  std::vector<uint64_t> swaps;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+
+  void free_host() {
+    if (h_acc) hipHostFree(h_acc);
+    if (h_cnt) hipHostFree(h_cnt);
+    if (done_ev) hipEventDestroy(done_ev);
+    h_acc = nullptr;
+    h_cnt = nullptr;
+    done_ev = nullptr;
+  }
+  ~Job() { free_host(); }
+};
+
+// Jobs of one (L, field) class: their stream, and the JobPtrs table and the
+// two task lists (each job at its own PT parity, and every parity flipped)
+// of the class's batched launch.
+struct Class {
+  int L = 0;
+  bool field = false;
+  hipStream_t st = nullptr;
+  std::vector<Job *> jobs;
+  bool dirty = true;
+  int64_t steps_since_build = 0;
+  JobPtrs *d_jt = nullptr;
+  TaskDesc *d_tasks[2] = {nullptr, nullptr};
+  size_t cap_jobs = 0, cap_tasks = 0;
+  JobPtrs *h_jt = nullptr;           // pinned staging
+  TaskDesc *h_tasks[2] = {nullptr, nullptr};
+  hipEvent_t up_ev = nullptr;        // the last upload from the staging
+  bool uploaded = false;
+  int ntasks[2] = {0, 0};
+
+  void free_all() {
+    hipFree(d_jt);
+    hipFree(d_tasks[0]);
+    hipFree(d_tasks[1]);
+    if (h_jt) hipHostFree(h_jt);
+    if (h_tasks[0]) hipHostFree(h_tasks[0]);
+    if (h_tasks[1]) hipHostFree(h_tasks[1]);
+    if (up_ev) hipEventDestroy(up_ev);
+    if (st) hipStreamDestroy(st);
+  }
+};
+
+}  // namespace
+
+struct sg_batch {
+  std::string err;
+  sg_batch_params bp{};
+  DevArena arena;
+  Clock::time_point t0;
+  std::map<int64_t, std::unique_ptr<Job>> jobs;
+  std::deque<Job *> queue;            // submitted, not admitted
+  std::vector<Class> classes;
+  std::deque<std::vector<hipEvent_t>> inflight;  // per step: one event per busy class
+  std::vector<hipEvent_t> ev_pool;
+  int64_t next_id = 0, steps = 0;
+  double flips = 0;
+
+  ~sg_batch() {
+    for (auto &s : inflight)
+      for (auto e : s) hipEventSynchronize(e);
+    for (auto &c : classes)
+      if (c.st) hipStreamSynchronize(c.st);
+    jobs.clear();  // job handles free into the arena before it goes
+    for (auto &s : inflight)
+      for (auto e : s) hipEventDestroy(e);
+    for (auto e : ev_pool) hipEventDestroy(e);
+    for (auto &c : classes) c.free_all();
+  }
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+  hipEvent_t take_event() {
+    if (!ev_pool.empty()) {
+      hipEvent_t e = ev_pool.back();
+      ev_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return e;
+  }
+
+  void create(const sg_batch_params &p) {
+    bp = p;
+    t0 = Clock::now();
+    HIPCHK(hipSetDevice(bp.device));
+    HIPCHK(arena.init((size_t)bp.arena_bytes));
+  }
+
+  Class &class_for(int L, bool field) {
+    for (auto &c : classes)
+      if (c.L == L && c.field == field) return c;
+    classes.emplace_back();
+    Class &c = classes.back();
+    c.L = L;
+    c.field = field;
+    HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c.up_ev, hipEventDisableTiming));
+    return c;
+  }
+
+  int class_index(const Class &c) const { return (int)(&c - classes.data()); }
+
+  // ------------------------------------------------------------ submission
+  int64_t submit(const sg_params &in, int64_t n_sweeps, double t_submit) {
+    const int pe = bp.pt_every;
+    if (in.pt_every != 0 && in.pt_every != pe)
+      throw Fail{SG_ERR_INVALID, "job pt_every differs from the engine's"};
+    if (n_sweeps <= 0 || n_sweeps % pe != 0)
+      throw Fail{SG_ERR_INVALID, "n_sweeps must be a positive multiple of pt_every"};
+    if (n_sweeps > 0xFFFFFFFFll) throw Fail{SG_ERR_INVALID, "n_sweeps too large"};
+    if (in.meas_every < 0 || in.meas_every % pe != 0)
+      throw Fail{SG_ERR_INVALID, "meas_every must be a multiple of pt_every"};
+    if (in.L < 2 || in.L > kMaxL || (in.L & 1) || !sweep_kernel_supported(in.L))
+      throw Fail{SG_ERR_INVALID, "L must be even and in [2, 48]"};
+    if (in.n_samples < 1 || in.n_temps < 1 || in.n_temps > 4096 || in.n_replicas < 1 ||
+        in.n_replicas > 8)
+      throw Fail{SG_ERR_INVALID, "bad sizes"};
+    if (!(in.H >= 0.0 && in.H <= 4.0)) throw Fail{SG_ERR_INVALID, "H out of range [0, 4]"};
+    if (!in.betas && !(in.T_min > 0.0 && in.T_max > 0.0))
+      throw Fail{SG_ERR_INVALID, "need betas or T_min/T_max > 0"};
+    auto j = std::make_unique<Job>();
+    j->id = next_id++;
+    j->p = in;
+    j->p.pt_every = pe;
+    j->p.device = bp.device;
+    j->p.form = SG_FORM_BATCHED;
+    if (in.betas) {
+      j->betas.assign(in.betas, in.betas + in.n_temps);
+      j->p.betas = j->betas.data();
+    }
+    j->n_sweeps = n_sweeps;
+    j->footprint = job_footprint(in.L, in.n_samples, in.n_replicas, in.n_temps);
+    if (j->footprint > arena.capacity())
+      throw Fail{SG_ERR_NOMEM, "job footprint exceeds the arena"};
+    j->t_submit = t_submit;
+    Job *raw = j.get();
+    jobs[raw->id] = std::move(j);
+    queue.push_back(raw);
+    return raw->id;
+  }
+
+  // ------------------------------------------------------------ admission
+  bool try_admit(Job &j) {
+    if (arena.capacity() - arena.in_use() < j.footprint) return false;
+    Class &c = class_for(j.p.L, j.p.H != 0.0);
+    auto h = std::make_unique<sg_handle>();
+    try {
+      h->create(j.p, &arena, c.st);
+    } catch (const Fail &f) {
+      if (f.code == SG_ERR_NOMEM) return false;  // fragmented: wait for a release
+      throw;
+    }
+    h->init_lattice();
+    h->launch_energy_all(h->energy_mode());
+    const size_t nc = (size_t)std::max(j.p.n_temps - 1, 1);
+    HIPCHK(hipHostMalloc((void **)&j.h_acc, sizeof(ObsAcc) * (size_t)h->G * h->T * 64));
+    HIPCHK(hipHostMalloc((void **)&j.h_cnt, sizeof(uint64_t) * nc));
+    HIPCHK(hipEventCreateWithFlags(&j.done_ev, hipEventDisableTiming));
+    j.h = std::move(h);
+    j.cls = class_index(c);
+    j.state = SG_JOB_RUNNING;
+    j.t_admit = now();
+    c.jobs.push_back(&j);
+    c.dirty = true;
+    return true;
+  }
+
+  void admit() {
+    while (!queue.empty()) {
+      if (!try_admit(*queue.front())) break;  // strict submission order
+      queue.pop_front();
+    }
+  }
+
+  // ------------------------------------------------------------ retirement
+  // A job whose final copies have landed gets its results and gives its
+  // device memory back.
+  void retire(bool wait) {
+    for (auto &kv : jobs) {
+      Job &j = *kv.second;
+      if (!j.finishing) continue;
+      if (wait) {
+        HIPCHK(hipEventSynchronize(j.done_ev));
+      } else {
+        const hipError_t q = hipEventQuery(j.done_ev);
+        if (q == hipErrorNotReady) continue;
+        HIPCHK(q);
+      }
+      sg_handle &h = *j.h;
+      j.res.assign((size_t)h.T * SG_NRES, 0.0);
+      h.results_from(j.h_acc, j.res.data());
+      j.swaps.assign(j.h_cnt, j.h_cnt + std::max(h.T - 1, 0));
+      j.h.reset();
+      j.free_host();
+      j.finishing = false;
+      j.state = SG_JOB_DONE;
+      j.t_done = now();
+    }
+  }
+
+  void finish(Job &j) {
+    sg_handle &h = *j.h;
+    HIPCHK(hipMemcpyAsync(j.h_acc, h.d_acc, sizeof(ObsAcc) * (size_t)h.G * h.T * 64,
+                          hipMemcpyDeviceToHost, h.st));
+    HIPCHK(hipMemcpyAsync(j.h_cnt, h.d_cnt, sizeof(uint64_t) * std::max(h.T - 1, 1),
+                          hipMemcpyDeviceToHost, h.st));
+    HIPCHK(hipEventRecord(j.done_ev, h.st));
+    j.finishing = true;
+  }
+
+  // ------------------------------------------------------------ a class's tables
+  void build(Class &c) {
+    size_t total = 0;
+    for (Job *j : c.jobs)
+      total += std::max(j->h->h_tasks[0].size(), j->h->h_tasks[1].size());
+    if (c.uploaded) HIPCHK(hipEventSynchronize(c.up_ev));  // staging free again
+    if (c.jobs.size() > c.cap_jobs || total > c.cap_tasks) {
+      // grow: queued launches may still read the old tables, so wait for them
+      HIPCHK(hipStreamSynchronize(c.st));
+      hipFree(c.d_jt); hipFree(c.d_tasks[0]); hipFree(c.d_tasks[1]);
+      if (c.h_jt) hipHostFree(c.h_jt);
+      if (c.h_tasks[0]) hipHostFree(c.h_tasks[0]);
+      if (c.h_tasks[1]) hipHostFree(c.h_tasks[1]);
+      c.d_jt = nullptr; c.d_tasks[0] = c.d_tasks[1] = nullptr;
+      c.h_jt = nullptr; c.h_tasks[0] = c.h_tasks[1] = nullptr;
+      c.cap_jobs = std::max<size_t>(2 * c.jobs.size(), 16);
+      c.cap_tasks = std::max<size_t>(2 * total, 1024);
+      HIPCHK(hipMalloc((void **)&c.d_jt, sizeof(JobPtrs) * c.cap_jobs));
+      HIPCHK(hipHostMalloc((void **)&c.h_jt, sizeof(JobPtrs) * c.cap_jobs));
+      for (int k = 0; k < 2; ++k) {
+        HIPCHK(hipMalloc((void **)&c.d_tasks[k], sizeof(TaskDesc) * c.cap_tasks));
+        HIPCHK(hipHostMalloc((void **)&c.h_tasks[k], sizeof(TaskDesc) * c.cap_tasks));
+      }
+    }
+    // list 0: every job at its current PT parity; list 1: every parity
+    // flipped.  Pair tasks of all jobs first, so the dispatcher's greedy
+    // fill ends evenly.
+    for (int k = 0; k < 2; ++k) {
+      size_t n = 0;
+      for (int pass = 0; pass < 2; ++pass)
+        for (size_t ji = 0; ji < c.jobs.size(); ++ji) {
+          sg_handle &h = *c.jobs[ji]->h;
+          const int par = (int)(((h.sweep / bp.pt_every) + k) & 1);
+          for (TaskDesc t : h.h_tasks[par]) {
+            const bool pair = t.lat[1] != 0xFFFFFFFFu;
+            if (pair != (pass == 0)) continue;
+            t.job = (uint32_t)ji;
+            c.h_tasks[k][n++] = t;
+          }
+        }
+      c.ntasks[k] = (int)n;
+    }
+    for (size_t ji = 0; ji < c.jobs.size(); ++ji) c.h_jt[ji] = c.jobs[ji]->h->self;
+    HIPCHK(hipMemcpyAsync(c.d_jt, c.h_jt, sizeof(JobPtrs) * c.jobs.size(),
+                          hipMemcpyHostToDevice, c.st));
+    for (int k = 0; k < 2; ++k)
+      HIPCHK(hipMemcpyAsync(c.d_tasks[k], c.h_tasks[k], sizeof(TaskDesc) * c.ntasks[k],
+                            hipMemcpyHostToDevice, c.st));
+    HIPCHK(hipEventRecord(c.up_ev, c.st));
+    c.uploaded = true;
+    c.dirty = false;
+    c.steps_since_build = 0;
+  }
+
+  // ------------------------------------------------------------ one step
+  // After each PT interval: host counters, measurements, and the final
+  // copies of jobs that reached their sweep count (which leave the class).
+  void after_interval(Class &c) {
+    const int pe = bp.pt_every;
+    bool left = false;
+    for (Job *jp : c.jobs) {
+      sg_handle &h = *jp->h;
+      const int par = (int)((h.sweep / pe) & 1);
+      for (int t = par; t + 1 < h.T; t += 2) h.swap_att[t] += (uint64_t)h.G * h.R * 64;
+      h.sweep += pe;
+      const double fl = (double)pe * h.N * 64.0 * h.G * h.R * h.T;
+      h.flips += fl;
+      flips += fl;
+      if (h.meas_every > 0 && h.sweep % h.meas_every == 0) h.measure(nullptr, nullptr);
+      if (h.sweep >= jp->n_sweeps) {
+        finish(*jp);
+        left = true;
+      }
+    }
+    if (left) {
+      c.jobs.erase(std::remove_if(c.jobs.begin(), c.jobs.end(),
+                                  [](Job *j) { return j->finishing; }),
+                   c.jobs.end());
+      c.dirty = true;
+    }
+  }
+
+  void run_class(Class &c) {
+    const int pe = bp.pt_every;
+    for (int i = 0; i < bp.slice_pt_steps && !c.jobs.empty(); ++i) {
+      const int emode = c.field ? 2 : 1;
+      if (bp.form == SG_FORM_BATCHED) {
+        if (c.dirty) build(c);
+        const int k = (int)(c.steps_since_build & 1);
+        HIPCHK(launch_block(c.L, c.field, c.d_jt, c.d_tasks[k], c.ntasks[k], 1, pe, emode, c.st));
+        HIPCHK(launch_advance(c.d_jt, (int)c.jobs.size(), pe, c.st));
+        ++c.steps_since_build;
+      } else {
+        for (Job *jp : c.jobs) {
+          sg_handle &h = *jp->h;
+          const int par = (int)((h.sweep / pe) & 1);
+          HIPCHK(launch_block(c.L, c.field, h.d_self, h.d_tasks[par],
+                              (int)h.h_tasks[par].size(), 1, pe, emode, c.st));
+          HIPCHK(launch_advance(h.d_self, 1, pe, c.st));
+        }
+      }
+      after_interval(c);
+    }
+  }
+
+  void step() {
+    retire(false);
+    admit();
+    std::vector<hipEvent_t> evs;
+    for (auto &c : classes) {
+      if (c.jobs.empty()) continue;
+      run_class(c);
+      hipEvent_t e = take_event();
+      HIPCHK(hipEventRecord(e, c.st));
+      evs.push_back(e);
+    }
+    ++steps;
+    if (!evs.empty()) inflight.push_back(std::move(evs));
+    // keep at most max_inflight steps queued on the device
+    while ((int)inflight.size() > std::max(bp.max_inflight, 1)) {
+      for (auto e : inflight.front()) {
+        HIPCHK(hipEventSynchronize(e));
+        ev_pool.push_back(e);
+      }
+      inflight.pop_front();
+    }
+    retire(false);
+  }
+
+  int n_running() const {
+    int n = 0;
+    for (auto &kv : jobs) n += kv.second->state == SG_JOB_RUNNING;
+    return n;
+  }
+
+  bool busy() const {
+    for (auto &kv : jobs)
+      if (kv.second->state != SG_JOB_DONE) return true;
+    return false;
+  }
+
+  void drain() {
+    while (busy()) {
+      bool any_active = !queue.empty();
+      for (auto &c : classes) any_active = any_active || !c.jobs.empty();
+      if (any_active)
+        step();
+      else
+        retire(true);
+    }
+  }
+
+  Job &job(int64_t id) {
+    auto it = jobs.find(id);
+    if (it == jobs.end()) throw Fail{SG_ERR_INVALID, "unknown job id"};
+    return *it->second;
+  }
+};
+
+// ---------------------------------------------------------------- C-ABI
+namespace {
+template <class F>
+sg_status bguarded(sg_batch *b, F &&f) {
+  if (!b) return SG_ERR_INVALID;
+  try {
+    f();
+    b->err.clear();
+    return SG_OK;
+  } catch (const Fail &e) {
+    b->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    b->err = "host allocation failed";
+    return SG_ERR_NOMEM;
+  }
+}
+thread_local std::string g_batch_create_error;
+}  // namespace
+
+extern "C" {
+
+void sg_batch_params_default(sg_batch_params *bp) {
+  std::memset(bp, 0, sizeof(*bp));
+  bp->device = 0;
+  bp->pt_every = 16;
+  bp->slice_pt_steps = 1;
+  bp->form = SG_FORM_BATCHED;
+  bp->arena_bytes = (int64_t)4 << 30;
+  bp->max_inflight = 2;
+}
+
+sg_status sg_batch_create(const sg_batch_params *bp, sg_batch **out) {
+  if (!out) return SG_ERR_INVALID;
+  *out = nullptr;
+  g_batch_create_error.clear();
+  auto bad = [](const char *m) { g_batch_create_error = m; return SG_ERR_INVALID; };
+  if (!bp) return bad("params is NULL");
+  if (bp->pt_every < 1) return bad("pt_every must be >= 1");
+  if (bp->slice_pt_steps < 1) return bad("slice_pt_steps must be >= 1");
+  if (bp->form != SG_FORM_PLAIN && bp->form != SG_FORM_BATCHED) return bad("unknown form");
+  if (bp->arena_bytes < (int64_t)DevArena::kAlign) return bad("arena too small");
+  sg_batch *b = new (std::nothrow) sg_batch();
+  if (!b) return bad("host allocation failed");
+  try {
+    b->create(*bp);
+  } catch (const Fail &e) {
+    g_batch_create_error = e.msg;
+    delete b;
+    return e.code;
+  }
+  *out = b;
+  return SG_OK;
+}
+
+sg_status sg_batch_destroy(sg_batch *b) {
+  if (!b) return SG_ERR_INVALID;
+  delete b;
+  return SG_OK;
+}
+
+const char *sg_batch_last_error(const sg_batch *b) {
+  return b ? b->err.c_str() : g_batch_create_error.c_str();
+}
+
+int64_t sg_job_footprint(const sg_params *p) {
+  if (!p || p->L < 2 || p->n_samples < 1 || p->n_temps < 1 || p->n_replicas < 1) return -1;
+  return (int64_t)job_footprint(p->L, p->n_samples, p->n_replicas, p->n_temps);
+}
+
+sg_status sg_submit(sg_batch *b, const sg_params *p, int64_t n_sweeps, int64_t *id) {
+  if (!p || !id) return SG_ERR_INVALID;
+  return bguarded(b, [&] { *id = b->submit(*p, n_sweeps, b->now()); });
+}
+
+sg_status sg_step(sg_batch *b, int32_t *n_running, int32_t *n_queued) {
+  return bguarded(b, [&] {
+    b->step();
+    if (n_running) *n_running = b->n_running();
+    if (n_queued) *n_queued = (int32_t)b->queue.size();
+  });
+}
+
+sg_status sg_drain(sg_batch *b) {
+  return bguarded(b, [&] { b->drain(); });
+}
+
+sg_status sg_poll(sg_batch *b, int64_t id, int32_t *state) {
+  if (!state) return SG_ERR_INVALID;
+  return bguarded(b, [&] {
+    Job &j = b->job(id);
+    if (j.finishing) b->retire(false);
+    *state = j.state;
+  });
+}
+
+sg_status sg_result(sg_batch *b, int64_t id, double *res, uint64_t *swaps_accepted) {
+  if (!res) return SG_ERR_INVALID;
+  return bguarded(b, [&] {
+    Job &j = b->job(id);
+    if (j.finishing) b->retire(false);
+    if (j.state != SG_JOB_DONE) throw Fail{SG_ERR_STATE, "job not finished"};
+    std::memcpy(res, j.res.data(), sizeof(double) * j.res.size());
+    if (swaps_accepted && !j.swaps.empty())
+      std::memcpy(swaps_accepted, j.swaps.data(), sizeof(uint64_t) * j.swaps.size());
+    b->jobs.erase(id);
+  });
+}
+
+sg_status sg_job_times(sg_batch *b, int64_t id, double *t_submit, double *t_admit,
+                       double *t_done) {
+  if (!t_submit || !t_admit || !t_done) return SG_ERR_INVALID;
+  return bguarded(b, [&] {
+    Job &j = b->job(id);
+    *t_submit = j.t_submit;
+    *t_admit = j.t_admit;
+    *t_done = j.t_done;
+  });
+}
+
+sg_status sg_batch_stats(sg_batch *b, double *flips, int64_t *steps, int64_t *arena_in_use,
+                         int64_t *arena_capacity) {
+  return bguarded(b, [&] {
+    if (flips) *flips = b->flips;
+    if (steps) *steps = b->steps;
+    if (arena_in_use) *arena_in_use = (int64_t)b->arena.in_use();
+    if (arena_capacity) *arena_capacity = (int64_t)b->arena.capacity();
+  });
+}
+
+sg_status sg_serve(sg_batch *b, const sg_params *jp, const int64_t *n_sweeps,
+                   const double *arrival_s, int32_t n_jobs, int64_t *ids, double *wall_s) {
+  if (!jp || !n_sweeps || !arrival_s || !ids || n_jobs < 0) return SG_ERR_INVALID;
+  return bguarded(b, [&] {
+    for (int32_t i = 1; i < n_jobs; ++i)
+      if (arrival_s[i] < arrival_s[i - 1])
+        throw Fail{SG_ERR_INVALID, "arrival times must be non-decreasing"};
+    const double start = b->now();
+    int32_t next = 0;
+    while (next < n_jobs || b->busy()) {
+      const double t = b->now() - start;
+      while (next < n_jobs && arrival_s[next] <= t) {
+        ids[next] = b->submit(jp[next], n_sweeps[next], start + arrival_s[next]);
+        ++next;
+      }
+      bool active = !b->queue.empty();
+      for (auto &c : b->classes) active = active || !c.jobs.empty();
+      if (active) {
+        b->step();
+      } else if (b->busy()) {
+        b->retire(false);
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+      } else if (next < n_jobs) {
+        const double wait = arrival_s[next] - (b->now() - start);
+        if (wait > 0)
+          std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 1e-3)));
+      }
+    }
+    if (wall_s) *wall_s = b->now() - start;
+  });
+}
+
+}  // extern "C"

@@ -70,6 +70,31 @@ __device__ __forceinline__ u32x4 philox_dev(uint32_t c0, uint32_t c1, uint32_t c
   return r;
 }
 
+// Address spaces of what a kernel reaches through its job table.  The table
+// holds generic pointers; accessed as such, every load and store would be a
+// flat access and no uniform load could become a scalar load.  Descriptors,
+// thresholds and counters are read-only during a launch and uniform across
+// the work-group: they are read through the constant address space (s_load
+// into SGPRs).  Spins and couplings are global.
+#define SG_GLOBAL __attribute__((address_space(1)))
+#define SG_CONST __attribute__((address_space(4)))
+
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "dword-sized records only");
+  const SG_CONST uint32_t *q = (const SG_CONST uint32_t *)p;
+  T v;
+  uint32_t *d = (uint32_t *)&v;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = __builtin_amdgcn_readfirstlane(q[i]);
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ SG_GLOBAL T *to_global(T *p) {
+  return (SG_GLOBAL T *)p;
+}
+
 template <int L>
 struct Geo {
   static constexpr int N = L * L * L;
@@ -283,11 +308,8 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 template <int L, bool FIELD>
 __global__ __launch_bounds__(Geo<L>::NT) void k_block(
-    uint64_t *__restrict__ spins, const uint64_t *__restrict__ Jg,
-    const LatDesc *__restrict__ descs, const TaskDesc *__restrict__ tasks,
-    const PairDesc *__restrict__ pairs, const ThrRec *__restrict__ thr,
-    const JobState *__restrict__ jobs, int do_pt, int nsweeps, int want_energy,
-    int32_t *__restrict__ energy, unsigned long long *__restrict__ counters) {
+    const JobPtrs *__restrict__ jt, const TaskDesc *__restrict__ tasks, int do_pt,
+    int nsweeps, int want_energy) {
   using G = Geo<L>;
   // colour 0 (black) and colour 1 (white) halves of the two lattices as
   // separate arrays: a phase writes one and reads the other, so the compiler
@@ -296,14 +318,19 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   __shared__ uint64_t red[G::NW * 24];
   __shared__ unsigned long long smask;
 
-  const TaskDesc tk = tasks[blockIdx.x];
+  const TaskDesc tk = ld_uniform(tasks + blockIdx.x);
+  const JobPtrs jp = ld_uniform(jt + tk.job);
+  SG_GLOBAL uint64_t *__restrict__ spins = to_global(jp.spins);
+  const SG_GLOBAL uint64_t *__restrict__ Jg = to_global(jp.J);
+  SG_GLOBAL int32_t *__restrict__ energy = to_global(jp.energy);
+  unsigned long long *__restrict__ counters = jp.counters;
   const bool two = tk.lat[1] != kNone;
-  const LatDesc d0 = descs[tk.lat[0]];
-  const LatDesc d1 = descs[two ? tk.lat[1] : tk.lat[0]];
+  const LatDesc d0 = ld_uniform(jp.descs + tk.lat[0]);
+  const LatDesc d1 = ld_uniform(jp.descs + (two ? tk.lat[1] : tk.lat[0]));
   const int tid = threadIdx.x;
   const bool active = tid < G::Q;
   {
-    const uint64_t *g0 = spins + d0.spin_off, *g1 = spins + d1.spin_off;
+    const SG_GLOBAL uint64_t *g0 = spins + d0.spin_off, *g1 = spins + d1.spin_off;
     for (int i = tid; i < G::NH; i += G::NT) {
       latB[0][i] = g0[i];
       latW[0][i] = g0[G::NH + i];
@@ -317,7 +344,7 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
 
   // neighbour indices within the opposite colour's half (y, z: same for both
   // colours; x: per colour) and the 6 couplings of each owned site
-  const uint64_t *Jx = Jg + d0.j_off, *Jy = Jx + G::N, *Jz = Jy + G::N;
+  const SG_GLOBAL uint64_t *Jx = Jg + d0.j_off, *Jy = Jx + G::N, *Jz = Jy + G::N;
   int yz[2][4], xi[2][2][2];
   uint64_t jw[2][2][6];
   if (active) {
@@ -350,23 +377,23 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   ThrRec tr0{}, tr1{};
   uint32_t n0 = 0;
   if (nsweeps > 0) {
-    tr0 = thr[d0.thr_idx];
-    tr1 = thr[d1.thr_idx];
-    n0 = jobs[d0.job].sweep;
+    tr0 = ld_uniform(jp.thr + d0.thr_idx);
+    tr1 = ld_uniform(jp.thr + d1.thr_idx);
+    n0 = ld_uniform(jp.js).sweep;
   }
   __syncthreads();
 
   // ---- PT exchange of the pair (t, t+1) at step b = sweep / pt_every
   if (do_pt && two && tk.pair != kNone) {
-    const PairDesc p = pairs[tk.pair];
-    const JobState js = jobs[p.job];
+    const PairDesc p = ld_uniform(jp.pairs + tk.pair);
+    const JobState js = ld_uniform(jp.js);
     if (tid < 64) {
       const uint32_t b = js.sweep / js.pt_every;
       const int l = tid;
       const uint32_t sw3 = (kTagPT << 28) | (p.rt & 0x0FFFFFFFu);
       const u32x4 r = philox_dev((uint32_t)(l >> 2), b, p.g, sw3, p.key0, p.key1);
       const uint32_t rr = r.v[l & 3];
-      const int32_t *Ea = energy + (size_t)p.e_a * 128, *Eb = energy + (size_t)p.e_b * 128;
+      const SG_GLOBAL int32_t *Ea = energy + (size_t)p.e_a * 128, *Eb = energy + (size_t)p.e_b * 128;
       const bool acc = pt_accept(p.beta_a, p.beta_b, Ea[l], Eb[l], Ea[64 + l], Eb[64 + l], p.H, rr);
       const unsigned long long m = __ballot(acc);
       if (l == 0) {
@@ -467,7 +494,7 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   }
 
   {
-    uint64_t *g0 = spins + d0.spin_off, *g1 = spins + d1.spin_off;
+    SG_GLOBAL uint64_t *g0 = spins + d0.spin_off, *g1 = spins + d1.spin_off;
     for (int i = tid; i < G::NH; i += G::NT) {
       g0[i] = latB[0][i];
       g0[G::NH + i] = latW[0][i];
@@ -531,24 +558,26 @@ __device__ __forceinline__ void site_geometry(int h, int c, int (&nb)[6], int &s
 
 template <int L, bool FIELD, bool IN_LDS>
 __global__ __launch_bounds__(1024) void k_big(
-    uint64_t *__restrict__ spins, const uint64_t *__restrict__ Jg,
-    const LatDesc *__restrict__ descs, const TaskDesc *__restrict__ tasks,
-    const PairDesc *__restrict__ pairs, const ThrRec *__restrict__ thr,
-    const JobState *__restrict__ jobs, int do_pt, int nsweeps, int want_energy,
-    int32_t *__restrict__ energy, unsigned long long *__restrict__ counters) {
+    const JobPtrs *__restrict__ jt, const TaskDesc *__restrict__ tasks, int do_pt,
+    int nsweeps, int want_energy) {
   using G = BigGeo<L>;
   constexpr int LDS_WORDS = IN_LDS ? 2 * G::N : 1;
   __shared__ uint64_t slat[LDS_WORDS];
   __shared__ uint64_t red[G::NW * 24];
   __shared__ unsigned long long smask;
 
-  const TaskDesc tk = tasks[blockIdx.x];
+  const TaskDesc tk = ld_uniform(tasks + blockIdx.x);
+  const JobPtrs jp = ld_uniform(jt + tk.job);
+  SG_GLOBAL uint64_t *__restrict__ spins = to_global(jp.spins);
+  const SG_GLOBAL uint64_t *__restrict__ Jg = to_global(jp.J);
+  SG_GLOBAL int32_t *__restrict__ energy = to_global(jp.energy);
+  unsigned long long *__restrict__ counters = jp.counters;
   const bool two = tk.lat[1] != kNone;
-  const LatDesc d0 = descs[tk.lat[0]];
-  const LatDesc d1 = descs[two ? tk.lat[1] : tk.lat[0]];
+  const LatDesc d0 = ld_uniform(jp.descs + tk.lat[0]);
+  const LatDesc d1 = ld_uniform(jp.descs + (two ? tk.lat[1] : tk.lat[0]));
   const int tid = threadIdx.x;
   const int nl = two ? 2 : 1;
-  uint64_t *gl[2] = {spins + d0.spin_off, spins + d1.spin_off};
+  uint64_t *gl[2] = {(uint64_t *)(spins + d0.spin_off), (uint64_t *)(spins + d1.spin_off)};
   uint64_t *lat[2];
   if (IN_LDS) {
     lat[0] = slat;
@@ -559,26 +588,26 @@ __global__ __launch_bounds__(1024) void k_big(
     lat[0] = gl[0];
     lat[1] = gl[1];
   }
-  const uint64_t *J = Jg + d0.j_off;
+  const SG_GLOBAL uint64_t *J = Jg + d0.j_off;
   ThrRec tr0{}, tr1{};
   uint32_t n0 = 0;
   if (nsweeps > 0) {
-    tr0 = thr[d0.thr_idx];
-    tr1 = thr[d1.thr_idx];
-    n0 = jobs[d0.job].sweep;
+    tr0 = ld_uniform(jp.thr + d0.thr_idx);
+    tr1 = ld_uniform(jp.thr + d1.thr_idx);
+    n0 = ld_uniform(jp.js).sweep;
   }
   __syncthreads();
 
   if (do_pt && two && tk.pair != kNone) {
-    const PairDesc p = pairs[tk.pair];
-    const JobState js = jobs[p.job];
+    const PairDesc p = ld_uniform(jp.pairs + tk.pair);
+    const JobState js = ld_uniform(jp.js);
     if (tid < 64) {
       const uint32_t b = js.sweep / js.pt_every;
       const int l = tid;
       const uint32_t sw3 = (kTagPT << 28) | (p.rt & 0x0FFFFFFFu);
       const u32x4 r = philox_dev((uint32_t)(l >> 2), b, p.g, sw3, p.key0, p.key1);
       const uint32_t rr = r.v[l & 3];
-      const int32_t *Ea = energy + (size_t)p.e_a * 128, *Eb = energy + (size_t)p.e_b * 128;
+      const SG_GLOBAL int32_t *Ea = energy + (size_t)p.e_a * 128, *Eb = energy + (size_t)p.e_b * 128;
       const bool acc = pt_accept(p.beta_a, p.beta_b, Ea[l], Eb[l], Ea[64 + l], Eb[64 + l], p.H, rr);
       const unsigned long long m = __ballot(acc);
       if (l == 0) {
@@ -690,53 +719,6 @@ __global__ __launch_bounds__(1024) void k_big(
   if (IN_LDS) {
     for (int a = 0; a < nl; ++a)
       for (int i = tid; i < G::N; i += G::NT) gl[a][i] = lat[a][i];
-  }
-}
-
-// --------------------------------------------------------------- PT kernel
-// One work-group per adjacent pair.  Wave 0 decides the 64 lanes (one
-// Philox word each), the ballot is the exchange mask, and the whole group
-// exchanges the masked bits of the two lattices.  Reference counterpart:
-// shuffle, eamsim.cu:43-90 (which swaps beta labels instead of spins).
-__global__ __launch_bounds__(256) void k_pt(uint64_t *__restrict__ spins,
-                                            int32_t *__restrict__ energy,
-                                            const PairDesc *__restrict__ pairs,
-                                            const JobState *__restrict__ jobs,
-                                            unsigned long long *__restrict__ counters) {
-  const PairDesc p = pairs[blockIdx.x];
-  const JobState js = jobs[p.job];
-  if (js.pt_every == 0) return;
-  const uint32_t b = js.sweep / js.pt_every;
-  if ((b & 1u) != p.parity) return;
-  __shared__ unsigned long long smask;
-  const int tid = threadIdx.x;
-  if (tid < 64) {
-    const int l = tid;
-    const uint32_t sw3 = (kTagPT << 28) | (p.rt & 0x0FFFFFFFu);
-    const u32x4 r = philox((uint32_t)(l >> 2), b, p.g, sw3, p.key0, p.key1);
-    const uint32_t rr = r.v[l & 3];
-    int32_t *Ea = energy + (size_t)p.e_a * 128, *Eb = energy + (size_t)p.e_b * 128;
-    const int32_t ua = Ea[l], ub = Eb[l], ka = Ea[64 + l], kb = Eb[64 + l];
-    const bool acc = pt_accept(p.beta_a, p.beta_b, ua, ub, ka, kb, p.H, rr);
-    const unsigned long long m = __ballot(acc);
-    if (acc) {
-      Ea[l] = ub; Eb[l] = ua;
-      Ea[64 + l] = kb; Eb[64 + l] = ka;
-    }
-    if (l == 0) {
-      smask = m;
-      atomicAdd(&counters[p.cnt_idx], (unsigned long long)__popcll(m));
-    }
-  }
-  __syncthreads();
-  const uint64_t m = smask;
-  if (m == 0) return;
-  uint64_t *A = spins + p.a_off, *B = spins + p.b_off;
-  for (uint32_t i = tid; i < p.n_words; i += 256) {
-    const uint64_t a = A[i], bb = B[i];
-    const uint64_t dd = (a ^ bb) & m;
-    A[i] = a ^ dd;
-    B[i] = bb ^ dd;
   }
 }
 
@@ -891,9 +873,9 @@ __global__ __launch_bounds__(256) void k_init_couplings(uint64_t *__restrict__ J
   J[j_off + (uint64_t)dir * N + s] = (uint64_t)r.v[0] | ((uint64_t)r.v[1] << 32);
 }
 
-__global__ void k_advance(JobState *jobs, int njobs, int nsweeps) {
+__global__ void k_advance(const JobPtrs *jt, int njobs, int nsweeps) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < njobs) jobs[i].sweep += (uint32_t)nsweeps;
+  if (i < njobs) jt[i].js->sweep += (uint32_t)nsweeps;
 }
 
 // ---------------------------------------------------------------- dispatch
@@ -910,23 +892,18 @@ bool sweep_kernel_supported(int L) {
   }
 }
 
-hipError_t launch_block(int L, bool field, uint64_t *spins, const uint64_t *J,
-                        const LatDesc *descs, const TaskDesc *tasks, int ntasks,
-                        const PairDesc *pairs, const ThrRec *thr, const JobState *jobs,
-                        int do_pt, int nsweeps, int want_energy, int32_t *energy,
-                        unsigned long long *counters, hipStream_t st) {
+hipError_t launch_block(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks,
+                        int ntasks, int do_pt, int nsweeps, int want_energy, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
   switch (L) {
 #define X(l)                                                                             \
   case l:                                                                                \
     if (field)                                                                           \
       hipLaunchKernelGGL((k_block<l, true>), dim3(ntasks), dim3(Geo<l>::NT), 0, st,      \
-                         spins, J, descs, tasks, pairs, thr, jobs, do_pt, nsweeps,       \
-                         want_energy, energy, counters);                                 \
+                         jt, tasks, do_pt, nsweeps, want_energy);                        \
     else                                                                                 \
       hipLaunchKernelGGL((k_block<l, false>), dim3(ntasks), dim3(Geo<l>::NT), 0, st,     \
-                         spins, J, descs, tasks, pairs, thr, jobs, do_pt, nsweeps,       \
-                         want_energy, energy, counters);                                 \
+                         jt, tasks, do_pt, nsweeps, want_energy);                        \
     return hipGetLastError();
     SG_LDS_SIZES(X)
 #undef X
@@ -934,12 +911,10 @@ hipError_t launch_block(int L, bool field, uint64_t *spins, const uint64_t *J,
   case l:                                                                                \
     if (field)                                                                           \
       hipLaunchKernelGGL((k_big<l, true, in_lds>), dim3(ntasks), dim3(1024), 0, st,      \
-                         spins, J, descs, tasks, pairs, thr, jobs, do_pt, nsweeps,       \
-                         want_energy, energy, counters);                                 \
+                         jt, tasks, do_pt, nsweeps, want_energy);                        \
     else                                                                                 \
       hipLaunchKernelGGL((k_big<l, false, in_lds>), dim3(ntasks), dim3(1024), 0, st,     \
-                         spins, J, descs, tasks, pairs, thr, jobs, do_pt, nsweeps,       \
-                         want_energy, energy, counters);                                 \
+                         jt, tasks, do_pt, nsweeps, want_energy);                        \
     return hipGetLastError();
     X(18, true) X(20, true) X(22, false) X(24, false) X(26, false) X(28, false)
     X(30, false) X(32, false) X(34, false) X(36, false) X(38, false) X(40, false)
@@ -948,14 +923,6 @@ hipError_t launch_block(int L, bool field, uint64_t *spins, const uint64_t *J,
     default:
       return hipErrorInvalidValue;
   }
-}
-
-hipError_t launch_pt(uint64_t *spins, int32_t *energy, const PairDesc *pairs, int npairs,
-                     const JobState *jobs, unsigned long long *counters, hipStream_t st) {
-  if (npairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pt, dim3(npairs), dim3(256), 0, st, spins, energy, pairs, jobs,
-                     counters);
-  return hipGetLastError();
 }
 
 hipError_t launch_measure(int L, int R, const uint64_t *spins, const uint64_t *J,
@@ -1001,9 +968,9 @@ hipError_t launch_init_couplings(int L, uint64_t *J, uint64_t j_off, uint32_t g,
   return hipGetLastError();
 }
 
-hipError_t launch_advance(JobState *jobs, int njobs, int nsweeps, hipStream_t st) {
+hipError_t launch_advance(const JobPtrs *jt, int njobs, int nsweeps, hipStream_t st) {
   if (njobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_advance, dim3((njobs + 255) / 256), dim3(256), 0, st, jobs, njobs,
+  hipLaunchKernelGGL(k_advance, dim3((njobs + 255) / 256), dim3(256), 0, st, jt, njobs,
                      nsweeps);
   return hipGetLastError();
 }

@@ -1,671 +1,8 @@
-// sg_runtime.cpp — host runtime and C-ABI of libsg (include/sg.h).
-//
-// Mirrors the reference's host object `eamsim` (Program/src/eamsim.cuh:66-110,
-// eamsim.cu:513-744): construction validates H and allocates, init_lattice
-// draws couplings and spins, run_simulation loops over measurement intervals,
-// save/load_checkpoint serialise the state.  Differences: every size is a
-// run-time parameter, the random numbers are counter-based (runs are
-// reproducible and resumable bit for bit), and errors are status codes.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <new>
-#include <string>
-#include <vector>
-
-#include "../../include/sg.h"
-#include "sg_common.h"
-#include "sg_kernels.h"
+// sg_runtime.cpp — C-ABI of libsg (include/sg.h) over sg_handle.
+#include "sg_handle.h"
 
 using namespace sg;
-
-namespace {
-
-thread_local std::string g_create_error;
-
-struct Fail {
-  sg_status code;
-  std::string msg;
-};
-
-#define HIPCHK(expr)                                                                  \
-  do {                                                                                \
-    hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess)                                                             \
-      throw Fail{SG_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)};  \
-  } while (0)
-
-template <class T>
-void dev_alloc(T **p, size_t count) {
-  *p = nullptr;
-  if (count == 0) count = 1;
-  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
-  if (e != hipSuccess) {
-    *p = nullptr;
-    throw Fail{SG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)};
-  }
-}
-
-uint64_t fnv1a(const void *data, size_t n, uint64_t h = 1469598103934665603ull) {
-  const unsigned char *p = (const unsigned char *)data;
-  for (size_t i = 0; i < n; ++i) {
-    h ^= p[i];
-    h *= 1099511628211ull;
-  }
-  return h;
-}
-
-}  // namespace
-
-struct sg_handle {
-  std::string err;
-  int L = 0, N = 0, G = 0, R = 0, T = 0, S = 0, pt_every = 0, meas_every = 0, form = 0;
-  int device = 0;
-  double H = 0;
-  std::vector<double> beta;
-  uint64_t cseed = 0, sseed = 0;
-  bool field = false;
-  hipStream_t st = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-
-  uint64_t *d_spins = nullptr, *d_J = nullptr;
-  ThrRec *d_thr = nullptr;
-  int32_t *d_energy = nullptr, *d_rec = nullptr;
-  LatDesc *d_lat = nullptr;
-  PairDesc *d_pairs = nullptr;
-  MeasDesc *d_meas = nullptr;
-  JobState *d_jobs = nullptr;
-  unsigned long long *d_cnt = nullptr;
-  ObsAcc *d_acc = nullptr;        // [group][temp][lane]
-  double *d_cs = nullptr;         // cos, sin of 2 pi c / L
-  int npairs = 0, nfield = 0;
-  // work-group tasks of the sweep kernel for PT steps of parity 0 and 1:
-  // pairs (t, t+1) with t of that parity, the other slots single; ordered by
-  // group (task_off[p][g] .. task_off[p][g+1])
-  TaskDesc *d_tasks[2] = {nullptr, nullptr};
-  std::vector<int> task_off[2];
-
-  // records for the sink: the measurement kernel writes d_rec, an async copy
-  // moves it to one of kRing pinned buffers, and the sink runs once the copy
-  // has landed (while later intervals already run on the device)
-  static constexpr int kRing = 3;
-  int32_t *h_ring[kRing] = {nullptr, nullptr, nullptr};
-  hipEvent_t ring_ev[kRing] = {nullptr, nullptr, nullptr};
-  struct PendingRec { int slot; int64_t sweep; };
-  std::vector<PendingRec> rec_queue;
-  int ring_next = 0;
-  size_t rec_count = 0;
-  std::vector<uint64_t> swap_att; // [T-1]
-  bool initialized = false, energy_valid = false;
-  int64_t sweep = 0, n_meas = 0;
-  double kernel_ms = 0, flips = 0;
-
-  // per-kernel profiling (sg_set_profiling)
-  struct KStat { int64_t launches = 0; double ms = 0, flips = 0; };
-  struct Pending { hipEvent_t a, b; int kind; double flips; };
-  bool profiling = false;
-  KStat kstat[3];
-  std::vector<Pending> pending;
-  std::vector<hipEvent_t> ev_pool;
-
-  hipEvent_t take_event() {
-    if (!ev_pool.empty()) {
-      hipEvent_t e = ev_pool.back();
-      ev_pool.pop_back();
-      return e;
-    }
-    hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
-    return e;
-  }
-  template <class F>
-  void timed(int kind, double fl, F &&launch) {
-    if (!profiling) {
-      launch();
-      return;
-    }
-    Pending p{take_event(), take_event(), kind, fl};
-    HIPCHK(hipEventRecord(p.a, st));
-    launch();
-    HIPCHK(hipEventRecord(p.b, st));
-    pending.push_back(p);
-  }
-  void collect_profile() {
-    if (pending.empty()) return;
-    HIPCHK(hipStreamSynchronize(st));
-    for (auto &p : pending) {
-      float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
-      kstat[p.kind].launches += 1;
-      kstat[p.kind].ms += ms;
-      kstat[p.kind].flips += p.flips;
-      ev_pool.push_back(p.a);
-      ev_pool.push_back(p.b);
-    }
-    pending.clear();
-  }
-
-  // batched form: one PT interval (sweep launch + counter advance) per
-  // (parity, length, energy) captured once as a hipGraph and replayed
-  struct GraphKey { int parity, seg, energy; hipGraphExec_t exec; };
-  std::vector<GraphKey> graphs;
-
-  ~sg_handle() { release(); }
-
-  void release() {
-    hipFree(d_spins); hipFree(d_J); hipFree(d_thr); hipFree(d_energy); hipFree(d_rec);
-    hipFree(d_lat); hipFree(d_pairs); hipFree(d_meas); hipFree(d_jobs); hipFree(d_cnt);
-    hipFree(d_acc); hipFree(d_cs); hipFree(d_tasks[0]); hipFree(d_tasks[1]);
-    d_tasks[0] = d_tasks[1] = nullptr;
-    for (auto &g : graphs) hipGraphExecDestroy(g.exec);
-    graphs.clear();
-    d_spins = d_J = nullptr; d_thr = nullptr; d_energy = d_rec = nullptr; d_lat = nullptr;
-    d_pairs = nullptr; d_meas = nullptr; d_jobs = nullptr; d_cnt = nullptr;
-    d_acc = nullptr; d_cs = nullptr;
-    for (int i = 0; i < kRing; ++i) {
-      if (h_ring[i]) hipHostFree(h_ring[i]);
-      if (ring_ev[i]) hipEventDestroy(ring_ev[i]);
-      h_ring[i] = nullptr;
-      ring_ev[i] = nullptr;
-    }
-    for (auto &p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
-    pending.clear();
-    for (auto e : ev_pool) hipEventDestroy(e);
-    ev_pool.clear();
-    if (ev0) hipEventDestroy(ev0);
-    if (ev1) hipEventDestroy(ev1);
-    if (st) hipStreamDestroy(st);
-    ev0 = ev1 = nullptr; st = nullptr;
-  }
-
-  size_t lat_index(int g, int rho, int t) const { return ((size_t)g * R + rho) * T + t; }
-  size_t n_lat() const { return (size_t)G * R * T; }
-
-  // ------------------------------------------------------------ set-up
-  void create(const sg_params &p) {
-    L = p.L; G = (p.n_samples + 63) / 64; S = p.n_samples; R = p.n_replicas; T = p.n_temps;
-    N = L * L * L; H = p.H; cseed = p.coupling_seed; sseed = p.spin_seed;
-    pt_every = p.pt_every; meas_every = p.meas_every; form = p.form; device = p.device;
-    field = H != 0.0;
-    beta.resize(T);
-    if (p.betas) {
-      for (int t = 0; t < T; ++t) beta[t] = p.betas[t];
-    } else {
-      for (int t = 0; t < T; ++t) {
-        const double f = T > 1 ? (double)t / (double)(T - 1) : 0.0;
-        beta[t] = 1.0 / (p.T_min * std::pow(p.T_max / p.T_min, f));
-      }
-    }
-    for (int t = 0; t < T; ++t)
-      if (!(beta[t] > 0.0) || !std::isfinite(beta[t]))
-        throw Fail{SG_ERR_INVALID, "every beta must be positive and finite"};
-
-    HIPCHK(hipSetDevice(device));
-    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    HIPCHK(hipEventCreate(&ev0));
-    HIPCHK(hipEventCreate(&ev1));
-    nfield = 2 * R + 1 + 3 * L;
-
-    dev_alloc(&d_spins, n_lat() * N);
-    dev_alloc(&d_J, (size_t)G * 3 * N);
-    dev_alloc(&d_thr, (size_t)T);
-    dev_alloc(&d_energy, n_lat() * 128);
-    dev_alloc(&d_rec, (size_t)G * T * 64 * nfield);
-    dev_alloc(&d_lat, n_lat());
-    dev_alloc(&d_meas, (size_t)G * T);
-    dev_alloc(&d_jobs, 1);
-    dev_alloc(&d_cnt, (size_t)std::max(T - 1, 1));
-
-    // thresholds
-    std::vector<ThrRec> thr(T);
-    for (int t = 0; t < T; ++t) build_thresholds(beta[t], thr[t]);
-    HIPCHK(hipMemcpy(d_thr, thr.data(), sizeof(ThrRec) * T, hipMemcpyHostToDevice));
-
-    // lattice descriptors [g][rho][t]
-    std::vector<LatDesc> ld(n_lat());
-    for (int g = 0; g < G; ++g)
-      for (int rho = 0; rho < R; ++rho)
-        for (int t = 0; t < T; ++t) {
-          const size_t i = lat_index(g, rho, t);
-          LatDesc &d = ld[i];
-          std::memset(&d, 0, sizeof(d));
-          d.spin_off = i * (uint64_t)N;
-          d.j_off = (uint64_t)g * 3 * N;
-          d.key0 = (uint32_t)sseed; d.key1 = (uint32_t)(sseed >> 32);
-          d.g = (uint32_t)g;
-          d.rt = ((uint32_t)rho << 16) | (uint32_t)t;
-          d.thr_idx = (uint32_t)t;
-          d.e_idx = (uint32_t)i;
-          d.job = 0;
-        }
-    HIPCHK(hipMemcpy(d_lat, ld.data(), sizeof(LatDesc) * ld.size(), hipMemcpyHostToDevice));
-
-    // PT pairs [g][rho][t], t = 0..T-2
-    npairs = G * R * std::max(T - 1, 0);
-    dev_alloc(&d_pairs, (size_t)npairs);
-    std::vector<PairDesc> pd;
-    pd.reserve(npairs);
-    for (int g = 0; g < G; ++g)
-      for (int rho = 0; rho < R; ++rho)
-        for (int t = 0; t + 1 < T; ++t) {
-          PairDesc d;
-          std::memset(&d, 0, sizeof(d));
-          d.a_off = lat_index(g, rho, t) * (uint64_t)N;
-          d.b_off = lat_index(g, rho, t + 1) * (uint64_t)N;
-          d.beta_a = beta[t]; d.beta_b = beta[t + 1]; d.H = H;
-          d.e_a = (uint32_t)lat_index(g, rho, t); d.e_b = (uint32_t)lat_index(g, rho, t + 1);
-          d.key0 = (uint32_t)sseed; d.key1 = (uint32_t)(sseed >> 32);
-          d.g = (uint32_t)g; d.rt = ((uint32_t)rho << 16) | (uint32_t)t;
-          d.job = 0; d.cnt_idx = (uint32_t)t; d.n_words = (uint32_t)N;
-          d.parity = (uint32_t)(t & 1);
-          pd.push_back(d);
-        }
-    if (npairs)
-      HIPCHK(hipMemcpy(d_pairs, pd.data(), sizeof(PairDesc) * npairs, hipMemcpyHostToDevice));
-
-    for (int par = 0; par < 2; ++par) {
-      std::vector<TaskDesc> tv;
-      task_off[par].assign(G + 1, 0);
-      for (int g = 0; g < G; ++g) {
-        task_off[par][g] = (int)tv.size();
-        for (int rho = 0; rho < R; ++rho)
-          for (int t = 0; t < T;) {
-            TaskDesc k;
-            k.pad = 0;
-            k.lat[0] = (uint32_t)lat_index(g, rho, t);
-            if ((t & 1) == par && t + 1 < T) {
-              k.lat[1] = (uint32_t)lat_index(g, rho, t + 1);
-              k.pair = (uint32_t)(((size_t)g * R + rho) * (T - 1) + t);
-              t += 2;
-            } else {
-              k.lat[1] = 0xFFFFFFFFu;
-              k.pair = 0xFFFFFFFFu;
-              t += 1;
-            }
-            tv.push_back(k);
-          }
-        // within a group, pairs (twice the work) first, then the single
-        // lattices, so that the dispatcher's greedy fill ends evenly
-        std::stable_partition(tv.begin() + task_off[par][g], tv.end(),
-                              [](const TaskDesc &k) { return k.lat[1] != 0xFFFFFFFFu; });
-      }
-      if (form == SG_FORM_BATCHED)  // one launch: all pairs of all groups first
-        std::stable_partition(tv.begin(), tv.end(),
-                              [](const TaskDesc &k) { return k.lat[1] != 0xFFFFFFFFu; });
-      task_off[par][G] = (int)tv.size();
-      dev_alloc(&d_tasks[par], tv.size());
-      HIPCHK(hipMemcpy(d_tasks[par], tv.data(), sizeof(TaskDesc) * tv.size(),
-                       hipMemcpyHostToDevice));
-    }
-
-    std::vector<MeasDesc> md((size_t)G * T);
-    for (int g = 0; g < G; ++g)
-      for (int t = 0; t < T; ++t) {
-        MeasDesc &d = md[(size_t)g * T + t];
-        d.spin_off = lat_index(g, 0, t) * (uint64_t)N;
-        d.rho_stride = (uint64_t)T * N;
-        d.j_off = (uint64_t)g * 3 * N;
-        d.out_off = ((uint64_t)g * T + t) * 64 * nfield;
-        d.acc_off = ((uint64_t)g * T + t) * 64;
-        d.cs_off = 0;
-      }
-    HIPCHK(hipMemcpy(d_meas, md.data(), sizeof(MeasDesc) * md.size(), hipMemcpyHostToDevice));
-
-    dev_alloc(&d_acc, (size_t)G * T * 64);
-    HIPCHK(hipMemset(d_acc, 0, sizeof(ObsAcc) * (size_t)G * T * 64));
-    std::vector<double> cs(2 * L);
-    for (int c = 0; c < L; ++c) {
-      cs[c] = std::cos(2.0 * M_PI * c / L);
-      cs[L + c] = std::sin(2.0 * M_PI * c / L);
-    }
-    dev_alloc(&d_cs, (size_t)2 * L);
-    HIPCHK(hipMemcpy(d_cs, cs.data(), sizeof(double) * 2 * L, hipMemcpyHostToDevice));
-    rec_count = (size_t)G * T * 64 * nfield;
-    swap_att.assign((size_t)std::max(T - 1, 1), 0);
-    reset_counters();
-  }
-
-  void ensure_ring() {
-    if (h_ring[0]) return;
-    for (int i = 0; i < kRing; ++i) {
-      hipError_t e = hipHostMalloc((void **)&h_ring[i], sizeof(int32_t) * rec_count);
-      if (e != hipSuccess) {
-        h_ring[i] = nullptr;
-        throw Fail{SG_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e)};
-      }
-      HIPCHK(hipEventCreateWithFlags(&ring_ev[i], hipEventDisableTiming));
-    }
-  }
-
-  void build_thresholds(double b, ThrRec &r) const {
-#pragma clang fp contract(off)
-    std::memset(&r, 0, sizeof(r));
-    for (int sig = 0; sig < 2; ++sig) {
-      const double sv = sig ? 1.0 : -1.0;
-      uint32_t nfin = 0;
-      for (int u = 0; u < 7; ++u) {
-        const double dE = (double)(12 - 4 * u) + 2.0 * H * sv;
-        if (dE <= 0.0) break;
-        const double p = det_exp(-b * dE);
-        const uint64_t thr = (uint64_t)(p * 4294967296.0);
-        if (thr >= (1ull << 32)) throw Fail{SG_ERR_INVALID, "threshold overflow"};
-        r.t[sig][u] = (uint32_t)thr;
-        nfin = (uint32_t)(u + 1);
-      }
-      r.nfin[sig] = nfin;
-    }
-    if (!field && (r.nfin[0] != 3 || r.nfin[1] != 3))
-      throw Fail{SG_ERR_INVALID, "unexpected threshold table at H = 0"};
-  }
-
-  void reset_counters() {
-    JobState js;
-    std::memset(&js, 0, sizeof(js));
-    js.sweep = (uint32_t)sweep;
-    js.pt_every = (uint32_t)(T > 1 ? pt_every : 0);
-    HIPCHK(hipMemcpy(d_jobs, &js, sizeof(js), hipMemcpyHostToDevice));
-  }
-
-  // ------------------------------------------------------------ lattice
-  void init_lattice() {
-    for (int g = 0; g < G; ++g)
-      HIPCHK(launch_init_couplings(L, d_J, (uint64_t)g * 3 * N, (uint32_t)g, (uint32_t)cseed,
-                                   (uint32_t)(cseed >> 32), st));
-    HIPCHK(launch_init_spins(L, d_spins, d_lat, (int)n_lat(), st));
-    HIPCHK(hipStreamSynchronize(st));
-    sweep = 0;
-    n_meas = 0;
-    HIPCHK(hipMemsetAsync(d_acc, 0, sizeof(ObsAcc) * (size_t)G * T * 64, st));
-    HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * std::max(T - 1, 1), st));
-    std::fill(swap_att.begin(), swap_att.end(), 0);
-    reset_counters();
-    energy_valid = false;
-    initialized = true;
-  }
-
-  // natural <-> checkerboard-split order
-  int cb(int s) const {
-    const int x = s % L, y = (s / L) % L, z = s / (L * L);
-    return ((x + y + z) & 1) * (N / 2) + (x >> 1) + (L / 2) * (y + L * z);
-  }
-
-  void get_spins(uint64_t *out) {
-    std::vector<uint64_t> raw(n_lat() * N);
-    HIPCHK(hipStreamSynchronize(st));
-    HIPCHK(hipMemcpy(raw.data(), d_spins, sizeof(uint64_t) * raw.size(), hipMemcpyDeviceToHost));
-    std::vector<int> perm(N);
-    for (int s = 0; s < N; ++s) perm[s] = cb(s);
-    for (size_t l = 0; l < n_lat(); ++l)
-      for (int s = 0; s < N; ++s) out[l * N + s] = raw[l * N + perm[s]];
-  }
-
-  void set_spins(const uint64_t *in) {
-    std::vector<uint64_t> raw(n_lat() * N);
-    std::vector<int> perm(N);
-    for (int s = 0; s < N; ++s) perm[s] = cb(s);
-    for (size_t l = 0; l < n_lat(); ++l)
-      for (int s = 0; s < N; ++s) raw[l * N + perm[s]] = in[l * N + s];
-    HIPCHK(hipStreamSynchronize(st));
-    HIPCHK(hipMemcpy(d_spins, raw.data(), sizeof(uint64_t) * raw.size(), hipMemcpyHostToDevice));
-    energy_valid = false;
-  }
-
-  // ------------------------------------------------------------ run loop
-  int energy_mode() const { return field ? 2 : 1; }   // K enters PT only when H != 0
-
-  // One sweep-kernel launch over the tasks of parity `par` (all groups in the
-  // batched form, one launch per group in the plain form), then the device
-  // sweep counter advance.  do_pt: the launch starts at a PT step.
-  void launch_interval(int par, int do_pt, int seg, int want_energy) {
-    const bool per_group = form == SG_FORM_PLAIN;
-    for (int g = 0; g < (per_group ? G : 1); ++g) {
-      const int a = per_group ? task_off[par][g] : 0;
-      const int b = per_group ? task_off[par][g + 1] : task_off[par][G];
-      const double fl = (double)seg * N * 64.0 * R * T * (per_group ? 1 : G);
-      timed(0, fl, [&] {
-        HIPCHK(launch_block(L, field, d_spins, d_J, d_lat, d_tasks[par] + a, b - a, d_pairs,
-                            d_thr, d_jobs, do_pt, seg, want_energy, d_energy, d_cnt, st));
-      });
-    }
-    if (seg > 0) HIPCHK(launch_advance(d_jobs, 1, seg, st));
-  }
-
-  void launch_energy_all(int mode) {
-    HIPCHK(launch_block(L, field, d_spins, d_J, d_lat, d_tasks[0], task_off[0][G], d_pairs,
-                        d_thr, d_jobs, 0, 0, mode, d_energy, d_cnt, st));
-  }
-
-  // Batched form: a full PT interval is one hipGraph per (parity, length,
-  // energy), replayed while the device sweep counter advances.
-  void run_graph_interval(int par, int seg, int want_energy) {
-    hipGraphExec_t exec = nullptr;
-    for (auto &k : graphs)
-      if (k.parity == par && k.seg == seg && k.energy == want_energy) exec = k.exec;
-    if (!exec) {
-      hipGraph_t g;
-      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      launch_interval(par, 1, seg, want_energy);
-      HIPCHK(hipStreamEndCapture(st, &g));
-      hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
-      HIPCHK(e);
-      graphs.push_back(GraphKey{par, seg, want_energy, exec});
-    }
-    HIPCHK(hipGraphLaunch(exec, st));
-  }
-
-  void run(int64_t n_sweeps, sg_obs_sink sink, void *user) {
-    if (!initialized) throw Fail{SG_ERR_STATE, "lattice not initialised"};
-    if (n_sweeps < 0) throw Fail{SG_ERR_INVALID, "negative sweep count"};
-    if (sweep + n_sweeps > 0xFFFFFFFFll) throw Fail{SG_ERR_INVALID, "sweep counter overflow"};
-    const bool pt = pt_every > 0 && T > 1;
-    HIPCHK(hipEventRecord(ev0, st));
-    int64_t remaining = n_sweeps;
-    while (remaining > 0) {
-      const int64_t n = sweep;
-      int64_t seg = remaining;
-      if (pt) seg = std::min<int64_t>(seg, pt_every - n % pt_every);
-      if (meas_every > 0) seg = std::min<int64_t>(seg, meas_every - n % meas_every);
-      const int64_t next = n + seg;
-      const bool want_energy = pt && next % pt_every == 0;
-      const bool pt_now = pt && n % pt_every == 0;
-      if (pt_now && !energy_valid) launch_energy_all(energy_mode());
-      const int par = pt_now ? (int)((n / pt_every) & 1) : 0;
-      const int emode = want_energy ? energy_mode() : 0;
-      if (form == SG_FORM_BATCHED && pt_now && seg == pt_every && !profiling) {
-        run_graph_interval(par, (int)seg, emode);
-      } else {
-        launch_interval(par, pt_now ? 1 : 0, (int)seg, emode);
-      }
-      if (pt_now)
-        for (int t = par; t + 1 < T; t += 2) swap_att[t] += (uint64_t)G * R * 64;
-      sweep = next;
-      energy_valid = want_energy;
-      flips += (double)seg * N * 64.0 * G * R * T;
-      remaining -= seg;
-      // records of earlier measurements whose copies have landed
-      if (!rec_queue.empty()) drain_records(sink, user, 0);
-      if (meas_every > 0 && next % meas_every == 0) measure(sink, user);
-    }
-    drain_records(sink, user, -1);
-    HIPCHK(hipEventRecord(ev1, st));
-    HIPCHK(hipEventSynchronize(ev1));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-    kernel_ms += ms;
-    collect_profile();
-  }
-
-  // The measurement folds itself into the device sums (ObsAcc); records leave
-  // the device only when a sink wants them.
-  void measure(sg_obs_sink sink, void *user) {
-    const bool want = sink != nullptr;
-    timed(2, 0.0, [&] {
-      HIPCHK(launch_measure(L, R, d_spins, d_J, d_meas, G * T, d_rec, nfield, d_acc, d_cs,
-                            want ? 1 : 0, st));
-    });
-    ++n_meas;
-    if (!want) return;
-    ensure_ring();
-    // a ring slot is reused only after its sink call has run
-    if ((int)rec_queue.size() == kRing) drain_records(sink, user, 1);
-    const int slot = ring_next;
-    ring_next = (ring_next + 1) % kRing;
-    HIPCHK(hipMemcpyAsync(h_ring[slot], d_rec, sizeof(int32_t) * rec_count,
-                          hipMemcpyDeviceToHost, st));
-    HIPCHK(hipEventRecord(ring_ev[slot], st));
-    rec_queue.push_back(PendingRec{slot, sweep});
-  }
-
-  // Runs the sink on copied records in measurement order: all of them with
-  // at_least < 0, otherwise at least that many plus any already landed.
-  void drain_records(sg_obs_sink sink, void *user, int at_least) {
-    size_t i = 0;
-    for (; i < rec_queue.size(); ++i) {
-      const PendingRec &p = rec_queue[i];
-      const bool must = at_least < 0 || (int)i < at_least;
-      if (!must) {
-        const hipError_t q = hipEventQuery(ring_ev[p.slot]);
-        if (q == hipErrorNotReady) break;
-        HIPCHK(q);
-      } else {
-        HIPCHK(hipEventSynchronize(ring_ev[p.slot]));
-      }
-      if (sink) sink(user, p.sweep, h_ring[p.slot], (int64_t)G * T * 64, nfield);
-    }
-    rec_queue.erase(rec_queue.begin(), rec_queue.begin() + (long)i);
-  }
-
-  void read_acc(std::vector<ObsAcc> &a) {
-    a.resize((size_t)G * T * 64);
-    HIPCHK(hipStreamSynchronize(st));
-    HIPCHK(hipMemcpy(a.data(), d_acc, sizeof(ObsAcc) * a.size(), hipMemcpyDeviceToHost));
-  }
-
-  // Per temperature, time means per sample from the exact sums, then the
-  // mean over the active samples.
-  void results(double *res) {
-    std::vector<ObsAcc> acc;
-    read_acc(acc);
-    const double Nd = N;
-    const double n = (double)n_meas;
-    for (int t = 0; t < T; ++t) {
-      double e = 0, m = 0, aq = 0, q2 = 0, q4 = 0, c1 = 0;
-      for (int s = 0; s < S && n_meas > 0; ++s) {
-        const ObsAcc &a = acc[((size_t)(s / 64) * T + t) * 64 + (s % 64)];
-        const double U = (double)a.sU / (n * R), K = (double)a.sK / (n * R);
-        const double M = 2.0 * K - Nd;
-        e += ((2.0 * U - 3.0 * Nd) - H * M) / Nd;
-        m += M / Nd;
-        aq += (double)a.sAbsQ / n / Nd;
-        q2 += (double)a.sQ2 / n / (Nd * Nd);
-        const unsigned __int128 q4s = ((unsigned __int128)a.sQ4hi << 64) | a.sQ4lo;
-        q4 += (double)q4s / n / (Nd * Nd * Nd * Nd);
-        c1 += a.sq1 / n;
-      }
-      const int64_t nm = n_meas;
-      double *r = res + (size_t)t * SG_NRES;
-      const double ns = S;
-      e /= ns; m /= ns; aq /= ns; q2 /= ns; q4 /= ns; c1 /= ns;
-      r[SG_RES_BETA] = beta[t];
-      r[SG_RES_E] = e;
-      r[SG_RES_M] = m;
-      r[SG_RES_Q] = aq;
-      r[SG_RES_Q2] = q2;
-      r[SG_RES_Q4] = q4;
-      r[SG_RES_BINDER] = q2 > 0 ? 0.5 * (3.0 - q4 / (q2 * q2)) : NAN;
-      r[SG_RES_CHI0] = q2;
-      r[SG_RES_CHI1] = c1;
-      r[SG_RES_XI_L] = (c1 > 0 && q2 / c1 > 1.0)
-                           ? std::sqrt(q2 / c1 - 1.0) / (2.0 * std::sin(M_PI / L)) / L
-                           : NAN;
-      r[SG_RES_NMEAS] = (double)nm;
-    }
-  }
-
-  // ------------------------------------------------------------ checkpoint
-  struct CkHeader {
-    char magic[8];
-    uint32_t version, L, G, R, T, S, pt_every, meas_every;
-    double H;
-    uint64_t cseed, sseed;
-    int64_t sweep, n_meas;
-    uint64_t payload_bytes;
-  };
-
-  size_t ck_size() const {
-    return sizeof(CkHeader) + sizeof(double) * T + sizeof(uint64_t) * n_lat() * N +
-           sizeof(uint64_t) * (size_t)G * 3 * N + sizeof(uint64_t) * 2 * std::max(T - 1, 1) +
-           sizeof(ObsAcc) * (size_t)G * T * 64 + sizeof(uint64_t);
-  }
-
-  void save(void *buf, size_t *size) {
-    const size_t need = ck_size();
-    if (!buf) { *size = need; return; }
-    if (*size < need) throw Fail{SG_ERR_INVALID, "checkpoint buffer too small"};
-    if (!initialized) throw Fail{SG_ERR_STATE, "lattice not initialised"};
-    char *p = (char *)buf;
-    CkHeader h;
-    std::memset(&h, 0, sizeof(h));
-    std::memcpy(h.magic, "SGCHKPT1", 8);
-    h.version = 2; h.L = L; h.G = G; h.R = R; h.T = T; h.S = S;
-    h.pt_every = pt_every; h.meas_every = meas_every; h.H = H; h.cseed = cseed; h.sseed = sseed;
-    h.sweep = sweep; h.n_meas = n_meas;
-    h.payload_bytes = need - sizeof(CkHeader) - sizeof(uint64_t);
-    std::memcpy(p, &h, sizeof(h)); p += sizeof(h);
-    std::memcpy(p, beta.data(), sizeof(double) * T); p += sizeof(double) * T;
-    get_spins((uint64_t *)p); p += sizeof(uint64_t) * n_lat() * N;
-    HIPCHK(hipMemcpy(p, d_J, sizeof(uint64_t) * (size_t)G * 3 * N, hipMemcpyDeviceToHost));
-    p += sizeof(uint64_t) * (size_t)G * 3 * N;
-    std::vector<uint64_t> cnt(std::max(T - 1, 1));
-    HIPCHK(hipMemcpy(cnt.data(), d_cnt, sizeof(uint64_t) * cnt.size(), hipMemcpyDeviceToHost));
-    std::memcpy(p, cnt.data(), sizeof(uint64_t) * cnt.size()); p += sizeof(uint64_t) * cnt.size();
-    std::memcpy(p, swap_att.data(), sizeof(uint64_t) * cnt.size()); p += sizeof(uint64_t) * cnt.size();
-    std::vector<ObsAcc> acc;
-    read_acc(acc);
-    std::memcpy(p, acc.data(), sizeof(ObsAcc) * acc.size()); p += sizeof(ObsAcc) * acc.size();
-    const uint64_t ck = fnv1a(buf, need - sizeof(uint64_t));
-    std::memcpy(p, &ck, sizeof(ck));
-    *size = need;
-  }
-
-  void load(const void *buf, size_t size) {
-    const size_t need = ck_size();
-    if (size != need) throw Fail{SG_ERR_FORMAT, "checkpoint size mismatch"};
-    const char *p = (const char *)buf;
-    CkHeader h;
-    std::memcpy(&h, p, sizeof(h));
-    if (std::memcmp(h.magic, "SGCHKPT1", 8) != 0 || h.version != 2)
-      throw Fail{SG_ERR_FORMAT, "checkpoint magic/version mismatch"};
-    uint64_t ck;
-    std::memcpy(&ck, p + need - sizeof(uint64_t), sizeof(ck));
-    if (ck != fnv1a(buf, need - sizeof(uint64_t)))
-      throw Fail{SG_ERR_FORMAT, "checkpoint checksum mismatch"};
-    if ((int)h.L != L || (int)h.G != G || (int)h.R != R || (int)h.T != T || (int)h.S != S ||
-        (int)h.pt_every != pt_every || (int)h.meas_every != meas_every || h.H != H ||
-        h.cseed != cseed || h.sseed != sseed)
-      throw Fail{SG_ERR_FORMAT, "checkpoint parameters differ from this handle"};
-    p += sizeof(h);
-    if (std::memcmp(p, beta.data(), sizeof(double) * T) != 0)
-      throw Fail{SG_ERR_FORMAT, "checkpoint ladder differs"};
-    p += sizeof(double) * T;
-    initialized = true;
-    set_spins((const uint64_t *)p); p += sizeof(uint64_t) * n_lat() * N;
-    HIPCHK(hipMemcpy(d_J, p, sizeof(uint64_t) * (size_t)G * 3 * N, hipMemcpyHostToDevice));
-    p += sizeof(uint64_t) * (size_t)G * 3 * N;
-    const size_t nc = std::max(T - 1, 1);
-    HIPCHK(hipMemcpy(d_cnt, p, sizeof(uint64_t) * nc, hipMemcpyHostToDevice)); p += sizeof(uint64_t) * nc;
-    std::memcpy(swap_att.data(), p, sizeof(uint64_t) * nc); p += sizeof(uint64_t) * nc;
-    HIPCHK(hipMemcpy(d_acc, p, sizeof(ObsAcc) * (size_t)G * T * 64, hipMemcpyHostToDevice));
-    sweep = h.sweep;
-    n_meas = h.n_meas;
-    reset_counters();
-    energy_valid = false;
-  }
-};
+using namespace sg::detail;
 
 // ---------------------------------------------------------------- C-ABI
 template <class F>

@@ -53,7 +53,21 @@ EXPORTS = [
     "sg_sweep_count", "sg_get_betas", "sg_get_spins", "sg_set_spins", "sg_get_couplings",
     "sg_set_couplings", "sg_get_energies", "sg_get_swap_counts", "sg_get_results",
     "sg_get_timing", "sg_set_profiling", "sg_get_kernel_stats",
+    "sg_batch_params_default", "sg_batch_create", "sg_batch_destroy", "sg_batch_last_error",
+    "sg_job_footprint", "sg_submit", "sg_step", "sg_drain", "sg_poll", "sg_result",
+    "sg_job_times", "sg_batch_stats", "sg_serve",
 ]
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
+
+
+class _BatchParams(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32), ("pt_every", ctypes.c_int32),
+        ("slice_pt_steps", ctypes.c_int32), ("form", ctypes.c_int32),
+        ("arena_bytes", ctypes.c_int64), ("max_inflight", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
 
 _lib = None
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -92,6 +106,22 @@ def load_library(path: str = LIB_PATH):
     L.sg_get_timing.argtypes = [_vp, _dp, _dp]
     L.sg_set_profiling.argtypes = [_vp, ctypes.c_int32]
     L.sg_get_kernel_stats.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), _dp, _dp]
+    _i64p = ctypes.POINTER(ctypes.c_int64)
+    L.sg_batch_params_default.argtypes = [ctypes.POINTER(_BatchParams)]
+    L.sg_batch_create.argtypes = [ctypes.POINTER(_BatchParams), ctypes.POINTER(_vp)]
+    L.sg_batch_destroy.argtypes = [_vp]
+    L.sg_batch_last_error.argtypes = [_vp]
+    L.sg_batch_last_error.restype = ctypes.c_char_p
+    L.sg_job_footprint.argtypes = [ctypes.POINTER(_Params)]
+    L.sg_job_footprint.restype = ctypes.c_int64
+    L.sg_submit.argtypes = [_vp, ctypes.POINTER(_Params), ctypes.c_int64, _i64p]
+    L.sg_step.argtypes = [_vp, _i32p, _i32p]
+    L.sg_drain.argtypes = [_vp]
+    L.sg_poll.argtypes = [_vp, ctypes.c_int64, _i32p]
+    L.sg_result.argtypes = [_vp, ctypes.c_int64, _dp, _u64p]
+    L.sg_job_times.argtypes = [_vp, ctypes.c_int64, _dp, _dp, _dp]
+    L.sg_batch_stats.argtypes = [_vp, _dp, _i64p, _i64p, _i64p]
+    L.sg_serve.argtypes = [_vp, ctypes.POINTER(_Params), _i64p, _dp, ctypes.c_int32, _i64p, _dp]
     _lib = L
     return L
 
@@ -102,6 +132,27 @@ def _check(status, h=None):
         raise SGError(f"libsg status {status}: {msg.decode() if msg else ''}")
 
 
+def _fill_params(p, L, n_samples, n_replicas, betas, T_range, H, coupling_seed, spin_seed,
+                 pt_every, meas_every, device=0, form=SG_FORM_PLAIN):
+    """Fills an sg_params; returns the betas array that p.betas points into
+    (keep it alive while p is used)."""
+    p.L, p.n_samples, p.n_replicas = L, n_samples, n_replicas
+    keep = None
+    if betas is not None:
+        keep = np.ascontiguousarray(betas, dtype=np.float64)
+        p.n_temps = len(keep)
+        p.betas = keep.ctypes.data_as(_dp)
+    else:
+        tmin, tmax, nt = T_range
+        p.n_temps = nt
+        p.T_min, p.T_max = tmin, tmax
+        p.betas = None
+    p.H = H
+    p.coupling_seed, p.spin_seed = coupling_seed, spin_seed
+    p.pt_every, p.meas_every, p.device, p.form = pt_every, meas_every, device, form
+    return keep
+
+
 class Sim:
     """One libsg handle: L, samples (64 per group), temperatures, replicas."""
 
@@ -110,19 +161,8 @@ class Sim:
                  meas_every=0, device=0, form=SG_FORM_PLAIN):
         lib = load_library()
         p = _Params()
-        p.L, p.n_samples, p.n_replicas = L, n_samples, n_replicas
-        if betas is not None:
-            self._betas = np.ascontiguousarray(betas, dtype=np.float64)
-            p.n_temps = len(self._betas)
-            p.betas = self._betas.ctypes.data_as(_dp)
-        else:
-            tmin, tmax, nt = T_range
-            self._betas = None
-            p.n_temps = nt
-            p.T_min, p.T_max = tmin, tmax
-        p.H = H
-        p.coupling_seed, p.spin_seed = coupling_seed, spin_seed
-        p.pt_every, p.meas_every, p.device, p.form = pt_every, meas_every, device, form
+        self._betas = _fill_params(p, L, n_samples, n_replicas, betas, T_range, H,
+                                   coupling_seed, spin_seed, pt_every, meas_every, device, form)
         h = _vp()
         _check(lib.sg_create(ctypes.byref(p), ctypes.byref(h)), None)
         self._h = h
@@ -241,6 +281,137 @@ class Sim:
     def load_checkpoint_bytes(self, data: bytes):
         buf = ctypes.create_string_buffer(data, len(data))
         self._c(load_library().sg_load_checkpoint(self._h, buf, len(data)))
+
+
+# ------------------------------------------------------------------ engine
+class Job:
+    """Parameters of one engine job (the arguments of Sim, plus n_sweeps)."""
+
+    def __init__(self, L, n_sweeps, n_samples=64, n_replicas=2, betas=None, T_range=None,
+                 H=0.0, coupling_seed=0x5EED0001, spin_seed=0x5EED0002, meas_every=0):
+        self.L, self.n_sweeps, self.n_samples, self.n_replicas = L, n_sweeps, n_samples, n_replicas
+        self.betas, self.T_range, self.H = betas, T_range, H
+        self.coupling_seed, self.spin_seed, self.meas_every = coupling_seed, spin_seed, meas_every
+
+    @property
+    def n_temps(self):
+        return len(self.betas) if self.betas is not None else self.T_range[2]
+
+    def params(self, pt_every=0):
+        p = _Params()
+        keep = _fill_params(p, self.L, self.n_samples, self.n_replicas, self.betas, self.T_range,
+                            self.H, self.coupling_seed, self.spin_seed, pt_every, self.meas_every)
+        return p, keep
+
+    def footprint(self):
+        p, _keep = self.params()
+        return load_library().sg_job_footprint(ctypes.byref(p))
+
+    def flips(self):
+        return float(self.L) ** 3 * 64 * ((self.n_samples + 63) // 64) * self.n_replicas * \
+            self.n_temps * self.n_sweeps
+
+
+class Batch:
+    """One sg_batch engine: jobs of mixed L resident on one device, admitted
+    by footprint, one sweep-kernel launch per (L, field) class and PT
+    interval (form SG_FORM_BATCHED) or per job (SG_FORM_PLAIN)."""
+
+    def __init__(self, pt_every=16, slice_pt_steps=1, form=SG_FORM_BATCHED,
+                 arena_bytes=4 << 30, max_inflight=2, device=0):
+        lib = load_library()
+        bp = _BatchParams()
+        lib.sg_batch_params_default(ctypes.byref(bp))
+        bp.device, bp.pt_every, bp.slice_pt_steps, bp.form = device, pt_every, slice_pt_steps, form
+        bp.arena_bytes, bp.max_inflight = arena_bytes, max_inflight
+        b = _vp()
+        st = lib.sg_batch_create(ctypes.byref(bp), ctypes.byref(b))
+        if st != 0:
+            msg = lib.sg_batch_last_error(None)
+            raise SGError(f"libsg status {st}: {msg.decode() if msg else ''}")
+        self._b = b
+        self.pt_every = pt_every
+        self._temps = {}
+
+    def close(self):
+        if getattr(self, "_b", None):
+            load_library().sg_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        self.close()
+
+    def _c(self, st):
+        if st != 0:
+            msg = load_library().sg_batch_last_error(self._b)
+            raise SGError(f"libsg status {st}: {msg.decode() if msg else ''}")
+
+    def submit(self, job: Job) -> int:
+        p, _keep = job.params(self.pt_every)
+        jid = ctypes.c_int64()
+        self._c(load_library().sg_submit(self._b, ctypes.byref(p), int(job.n_sweeps),
+                                         ctypes.byref(jid)))
+        self._temps[jid.value] = job.n_temps
+        return jid.value
+
+    def step(self):
+        nr, nq = ctypes.c_int32(), ctypes.c_int32()
+        self._c(load_library().sg_step(self._b, ctypes.byref(nr), ctypes.byref(nq)))
+        return nr.value, nq.value
+
+    def drain(self):
+        self._c(load_library().sg_drain(self._b))
+
+    def poll(self, jid: int) -> int:
+        s = ctypes.c_int32()
+        self._c(load_library().sg_poll(self._b, int(jid), ctypes.byref(s)))
+        return s.value
+
+    def times(self, jid: int):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._c(load_library().sg_job_times(self._b, int(jid), ctypes.byref(a), ctypes.byref(b),
+                                            ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def result(self, jid: int):
+        """(results [T][NRES], accepted swaps [T-1]); releases the job."""
+        T = self._temps[jid]
+        r = np.zeros((T, NRES), dtype=np.float64)
+        sw = np.zeros(max(T - 1, 1), dtype=np.uint64)
+        self._c(load_library().sg_result(self._b, int(jid), r.ctypes.data_as(_dp),
+                                         sw.ctypes.data_as(_u64p)))
+        del self._temps[jid]
+        return r, sw[: T - 1]
+
+    def stats(self):
+        fl = ctypes.c_double()
+        st, used, cap = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._c(load_library().sg_batch_stats(self._b, ctypes.byref(fl), ctypes.byref(st),
+                                              ctypes.byref(used), ctypes.byref(cap)))
+        return {"flips": fl.value, "steps": st.value, "arena_in_use": used.value,
+                "arena_capacity": cap.value}
+
+    def serve(self, jobs, arrivals):
+        """Runs the native serving loop over jobs arriving at `arrivals`
+        seconds; returns (ids, wall seconds)."""
+        n = len(jobs)
+        arr = (_Params * n)()
+        keep = []
+        for i, j in enumerate(jobs):
+            p, k = j.params(self.pt_every)
+            arr[i] = p
+            keep.append(k)
+        sweeps = np.ascontiguousarray([j.n_sweeps for j in jobs], dtype=np.int64)
+        at = np.ascontiguousarray(arrivals, dtype=np.float64)
+        ids = np.zeros(n, dtype=np.int64)
+        wall = ctypes.c_double()
+        _i64p = ctypes.POINTER(ctypes.c_int64)
+        self._c(load_library().sg_serve(self._b, arr, sweeps.ctypes.data_as(_i64p),
+                                         at.ctypes.data_as(_dp), n, ids.ctypes.data_as(_i64p),
+                                         ctypes.byref(wall)))
+        for i, j in zip(ids, jobs):
+            self._temps[int(i)] = j.n_temps
+        return ids, wall.value
 
 
 # ------------------------------------------------------------------ drop-in

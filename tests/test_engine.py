@@ -1,0 +1,155 @@
+"""The batch / serving engine (sg_batch_*): every job's results are the same
+alone and in a mix, bit for bit — whatever else runs beside it, whatever
+the admission order, arena size or launch form — and match the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+import sg
+from helpers import geometric_betas, results_from_records
+
+PT = 4
+
+
+def _mix():
+    """Mixed L, ladders, replica counts, sample counts, seeds, fields and
+    sweep counts (all multiples of the PT cadence)."""
+    return [
+        sg.Job(L=8, n_sweeps=24, n_samples=64, betas=geometric_betas(0.5, 2.0, 4),
+               coupling_seed=11, spin_seed=12, meas_every=8),
+        sg.Job(L=12, n_sweeps=16, n_samples=100, betas=geometric_betas(0.2, 2.0, 6),
+               coupling_seed=21, spin_seed=22, meas_every=4),
+        sg.Job(L=16, n_sweeps=12, n_samples=64, T_range=(0.3, 2.0, 5),
+               coupling_seed=31, spin_seed=32, meas_every=12),
+        sg.Job(L=8, n_sweeps=40, n_samples=130, betas=geometric_betas(0.4, 1.5, 3), H=0.25,
+               coupling_seed=41, spin_seed=42, meas_every=8),
+        sg.Job(L=20, n_sweeps=8, n_samples=64, betas=geometric_betas(0.3, 2.0, 4),
+               coupling_seed=51, spin_seed=52, meas_every=4),
+        sg.Job(L=24, n_sweeps=8, n_samples=64, n_replicas=3, betas=geometric_betas(0.3, 2.0, 3),
+               coupling_seed=61, spin_seed=62, meas_every=8),
+        sg.Job(L=12, n_sweeps=20, n_samples=64, betas=geometric_betas(0.2, 2.0, 7),
+               coupling_seed=71, spin_seed=72, meas_every=20),
+        sg.Job(L=4, n_sweeps=32, n_samples=64, betas=geometric_betas(0.5, 2.0, 2),
+               coupling_seed=81, spin_seed=82, meas_every=4),
+    ]
+
+
+def _alone(job):
+    sim = sg.Sim(L=job.L, n_samples=job.n_samples, n_replicas=job.n_replicas, betas=job.betas,
+                 T_range=job.T_range, H=job.H, coupling_seed=job.coupling_seed,
+                 spin_seed=job.spin_seed, pt_every=PT, meas_every=job.meas_every)
+    sim.keep_records = False
+    sim.init_lattice()
+    sim.run(job.n_sweeps)
+    res = sim.results()
+    acc, _ = sim.swap_counts()
+    sim.close()
+    return res, acc
+
+
+@pytest.fixture(scope="module")
+def alone():
+    return [_alone(j) for j in _mix()]
+
+
+def _check_same(got, ref):
+    for (r, s), (r0, s0) in zip(got, ref):
+        np.testing.assert_array_equal(r, r0)
+        np.testing.assert_array_equal(s, s0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", [sg.SG_FORM_BATCHED, sg.SG_FORM_PLAIN])
+@pytest.mark.parametrize("slice_steps", [1, 3])
+def test_mix_matches_each_job_alone(alone, form, slice_steps):
+    b = sg.Batch(pt_every=PT, slice_pt_steps=slice_steps, form=form, arena_bytes=256 << 20)
+    jobs = _mix()
+    ids = [b.submit(j) for j in jobs[:4]]
+    b.step()                       # the rest arrive while the first run
+    ids += [b.submit(j) for j in jobs[4:]]
+    b.drain()
+    assert all(b.poll(i) == sg.JOB_DONE for i in ids)
+    _check_same([b.result(i) for i in ids], alone)
+    assert b.stats()["arena_in_use"] == 0
+    b.close()
+
+
+@pytest.mark.gpu
+def test_small_arena_queues_jobs_and_keeps_results(alone):
+    jobs = _mix()
+    biggest = max(j.footprint() for j in jobs)
+    b = sg.Batch(pt_every=PT, arena_bytes=int(biggest * 1.5))
+    ids = [b.submit(j) for j in jobs]
+    running, queued = b.step()
+    assert queued > 0               # admission is limited by the footprint
+    b.drain()
+    _check_same([b.result(i) for i in ids], alone)
+    b.close()
+
+
+@pytest.mark.gpu
+def test_engine_job_matches_oracle():
+    betas = geometric_betas(0.5, 2.0, 4)
+    job = sg.Job(L=8, n_sweeps=24, n_samples=80, betas=betas, coupling_seed=5, spin_seed=6,
+                 meas_every=8)
+    b = sg.Batch(pt_every=PT, arena_bytes=64 << 20)
+    other = b.submit(sg.Job(L=12, n_sweeps=32, n_samples=64, betas=betas, meas_every=8))
+    jid = b.submit(job)
+    b.drain()
+    res, acc = b.result(jid)
+    b.result(other)
+    orc = oracle.OracleSim(8, 2, 2, betas, coupling_seed=5, spin_seed=6, pt_every=PT,
+                           meas_every=8)
+    orc.init()
+    orc.run(24)
+    ref = results_from_records([orc.meas(i) for i in range(orc.n_meas())], 8, 2, 0.0, 80, betas)
+    np.testing.assert_allclose(res, ref, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(acc, orc.swap_counts()[0])
+    b.close()
+
+
+@pytest.mark.gpu
+def test_serve_loop_completes_every_job(alone):
+    jobs = _mix()
+    arrivals = np.cumsum(np.full(len(jobs), 0.002))
+    b = sg.Batch(pt_every=PT, arena_bytes=256 << 20)
+    ids, wall = b.serve(jobs, arrivals)
+    assert wall > arrivals[-1]
+    for i, a in zip(ids, arrivals):
+        t_sub, t_adm, t_done = b.times(i)
+        assert t_sub <= t_adm <= t_done
+    t0 = b.times(ids[0])[0] - arrivals[0]
+    for i, a in zip(ids, arrivals):
+        assert abs(b.times(i)[0] - (t0 + a)) < 1e-9
+    _check_same([b.result(i) for i in ids], alone)
+    b.close()
+
+
+def test_engine_rejects_bad_parameters_before_any_device_call():
+    with pytest.raises(sg.SGError, match="pt_every"):
+        sg.Batch(pt_every=0)
+    with pytest.raises(sg.SGError, match="slice_pt_steps"):
+        sg.Batch(slice_pt_steps=0)
+    with pytest.raises(sg.SGError, match="arena"):
+        sg.Batch(arena_bytes=16)
+
+
+@pytest.mark.gpu
+def test_engine_validates_jobs():
+    b = sg.Batch(pt_every=PT, arena_bytes=1 << 20)
+    with pytest.raises(sg.SGError, match="multiple"):
+        b.submit(sg.Job(L=8, n_sweeps=10, betas=[1.0, 0.5]))
+    with pytest.raises(sg.SGError, match="exceeds"):
+        b.submit(sg.Job(L=48, n_sweeps=4, n_samples=64, betas=geometric_betas(0.3, 2.0, 8)))
+    with pytest.raises(sg.SGError, match="L must"):
+        b.submit(sg.Job(L=7, n_sweeps=4, betas=[1.0]))
+    b.close()
+
+
+def test_job_footprint_counts_every_block():
+    sg.load_library()
+    j = sg.Job(L=16, n_sweeps=16, n_samples=128, betas=geometric_betas(0.2, 2.0, 24))
+    N, G, R, T = 16 ** 3, 2, 2, 24
+    spins = 8 * G * R * T * N
+    J = 8 * G * 3 * N
+    assert spins + J < j.footprint() < 1.2 * (spins + J)
