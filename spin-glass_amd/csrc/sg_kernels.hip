@@ -216,6 +216,146 @@ __device__ int32_t wg_sum_planes(const uint64_t (&A0)[P0], uint64_t *red) {
   return v;
 }
 
+// Bit-sliced S (NS planes) += B (NB planes), ripple carry; the caller sizes
+// NS so that the sum cannot overflow.
+template <int NS, int NA, int NB>
+__device__ __forceinline__ void vc_add(uint64_t (&S)[NA], const uint64_t (&B)[NB]) {
+  static_assert(NS <= NA && NB <= NS, "plane counts");
+  uint64_t c = 0;
+#pragma unroll
+  for (int p = 0; p < NS; ++p) {
+    const uint64_t a = S[p];
+    if (p < NB) {
+      S[p] = a ^ B[p] ^ c;
+      c = maj3(a, B[p], c);
+    } else {
+      S[p] = a ^ c;
+      c = a & c;
+    }
+  }
+}
+
+constexpr int ceil_log2(int n) { return n <= 1 ? 0 : 1 + ceil_log2((n + 1) / 2); }
+
+// 32-bit DPP fetch / bit-sliced add (see dpp64 / dpp_add)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, true);
+}
+
+template <int NP, int CTRL, int ROW_MASK, int NA>
+__device__ __forceinline__ void dpp_add32(uint32_t (&A)[NA]) {
+  static_assert(NP < NA, "plane array too small");
+  uint32_t B[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) B[p] = dpp32<CTRL, ROW_MASK>(A[p]);
+  uint32_t carry = 0;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const uint32_t a = A[p];
+    A[p] = SG_BITOP3(a, B[p], carry, kXor3);
+    carry = SG_BITOP3(a, B[p], carry, kMaj3);
+  }
+  A[NP] = carry;
+}
+
+// Work-group total, per lane (sample), of the bit-sliced per-thread counts A
+// of NL lattices (P0 planes each).  A 64-bit DPP tree level costs ~6 VALU ops
+// per plane on every thread, so:
+//  1. the counts are folded through LDS onto the first TW <= 4 waves (one per
+//     SIMD), each folded thread adding K entries with ripple adds;
+//  2. v_permlane32_swap splits every 64-bit plane: thread t < 32 keeps
+//     samples 0..31 (its low dword plus thread t+32's), thread t >= 32
+//     samples 32..63, so the rest of the tree runs on 32-bit planes;
+//  3. a 32-bit DPP tree within each half (row_shr 1, 2, 4, 8, row_bcast 15)
+//     leaves the half totals in lanes 31 and 63;
+//  4. each lane reads its half's planes (readlane) and extracts its own bit:
+//     one integer per lane per tree wave, which wave 0 adds.
+// fold must hold NL*P0*NT words and red TW*NL*64 int32, both LDS.  Every
+// thread calls it; out is valid in wave 0 (lane = sample).
+template <int P0, int NT, int NL>
+__device__ void wg_count_lanes(const uint64_t (&A)[NL][P0], uint64_t *fold, int32_t *red,
+                               int32_t (&out)[NL]) {
+  constexpr int NW = NT / 64;
+  constexpr int TW = NW < 4 ? NW : 4;
+  constexpr int FT = TW * 64;
+  constexpr int K = (NT + FT - 1) / FT;
+  constexpr int PF = P0 + ceil_log2(K);   // planes after the fold
+  constexpr int PH = PF + 1;              // after the 32-lane split
+  constexpr int PW = PH + 5;              // after the half-wave tree
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int a = 0; a < NL; ++a)
+#pragma unroll
+    for (int p = 0; p < P0; ++p) fold[(a * P0 + p) * NT + tid] = A[a][p];
+  __syncthreads();
+  if (wave < TW) {
+    uint32_t H[NL][PW];
+#pragma unroll
+    for (int a = 0; a < NL; ++a) {
+      uint64_t S[PF];
+#pragma unroll
+      for (int p = 0; p < PF; ++p) S[p] = p < P0 ? fold[(a * P0 + p) * NT + tid] : 0;
+#pragma unroll
+      for (int k = 1; k < K; ++k) {
+        const int i = tid + k * FT;
+        if (i < NT) {
+          uint64_t B[P0];
+#pragma unroll
+          for (int p = 0; p < P0; ++p) B[p] = fold[(a * P0 + p) * NT + i];
+          vc_add<PF>(S, B);
+        }
+      }
+      // split: x = own low dwords, y = own high dwords; after the swap,
+      // lanes < 32 hold (own low, low of lane + 32) and lanes >= 32
+      // (high of lane - 32, own high)
+      uint32_t carry = 0;
+#pragma unroll
+      for (int p = 0; p < PF; ++p) {
+        const auto r = __builtin_amdgcn_permlane32_swap((uint32_t)S[p], (uint32_t)(S[p] >> 32),
+                                                        false, false);
+        const uint32_t x = r[0], y = r[1];
+        H[a][p] = SG_BITOP3(x, y, carry, kXor3);
+        carry = SG_BITOP3(x, y, carry, kMaj3);
+      }
+      H[a][PF] = carry;
+#pragma unroll
+      for (int p = PH; p < PW; ++p) H[a][p] = 0;
+    }
+#pragma unroll
+    for (int a = 0; a < NL; ++a) dpp_add32<PH + 0, 0x111, 0xF>(H[a]);
+#pragma unroll
+    for (int a = 0; a < NL; ++a) dpp_add32<PH + 1, 0x112, 0xF>(H[a]);
+#pragma unroll
+    for (int a = 0; a < NL; ++a) dpp_add32<PH + 2, 0x114, 0xF>(H[a]);
+#pragma unroll
+    for (int a = 0; a < NL; ++a) dpp_add32<PH + 3, 0x118, 0xF>(H[a]);
+#pragma unroll
+    for (int a = 0; a < NL; ++a) dpp_add32<PH + 4, 0x142, 0xA>(H[a]);
+#pragma unroll
+    for (int a = 0; a < NL; ++a) {
+      int32_t v = 0;
+#pragma unroll
+      for (int p = 0; p < PW; ++p) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)H[a][p], 31);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)H[a][p], 63);
+        const uint32_t word = lane < 32 ? lo : hi;
+        v |= (int32_t)((word >> (lane & 31)) & 1u) << p;
+      }
+      red[(wave * NL + a) * 64 + lane] = v;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < NL; ++a) {
+    int32_t v = 0;
+    if (wave == 0)
+#pragma unroll
+      for (int w = 0; w < TW; ++w) v += red[(w * NL + a) * 64 + lane];
+    out[a] = v;
+  }
+}
+
 // lanes whose bit-sliced count (u2 u1 u0) is >= k (k in 0..7, per thread)
 __device__ __forceinline__ uint64_t count_ge(uint64_t u2, uint64_t u1, uint64_t u0, int k) {
   const uint64_t k2 = 0 - (uint64_t)((k >> 2) & 1);
@@ -315,7 +455,8 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   // separate arrays: a phase writes one and reads the other, so the compiler
   // may reorder the loads and stores of a phase freely
   __shared__ uint64_t latB[2][G::NH], latW[2][G::NH];
-  __shared__ uint64_t red[G::NW * 24];
+  __shared__ uint64_t fold[2 * 4 * G::NT];   // energy reduction scratch
+  __shared__ int32_t redi[4 * 2 * 64];
   __shared__ unsigned long long smask;
 
   const TaskDesc tk = ld_uniform(tasks + blockIdx.x);
@@ -451,44 +592,33 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
     }
   }
 
-  if (want_energy) {
-    // every bond touches exactly one black site (colour 0); the bit-sliced
-    // counts of both lattices are formed first, so the couplings are dead
-    // before the reductions
-    uint64_t AU[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-    uint64_t AK[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    if (active) {
-#pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        if (a == 1 && !two) break;
-        const uint64_t *opp = latW[a];
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const uint64_t s = latB[a][tid + q * G::Q];
-          uint64_t nv[6];
-          nv[0] = opp[xi[q][0][0]];
-          nv[1] = opp[xi[q][0][1]];
-          nv[2] = opp[yz[q][0]];
-          nv[3] = opp[yz[q][1]];
-          nv[4] = opp[yz[q][2]];
-          nv[5] = opp[yz[q][3]];
-          uint64_t u0, u1, u2;
-          bond_count(s, nv, jw[q][0], u0, u1, u2);
-          vc_add3(AU[a], u0, u1, u2);
-          vc_add1(AK[a], s);
-          vc_add1(AK[a], latW[a][tid + q * G::Q]);
-        }
-      }
-    }
+  // energies of the final states: every bond touches exactly one black site
+  // (colour 0).  The bit-sliced counts are formed first (registers), the
+  // lattices leave for global memory, then the counts are reduced.
+  uint64_t AU[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  uint64_t AK[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  if (want_energy && active) {
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
       if (a == 1 && !two) break;
-      const LatDesc &d = a == 0 ? d0 : d1;
-      const int32_t U = wg_sum_planes<4, G::NW>(AU[a], red);
-      if (tid < 64) energy[(size_t)d.e_idx * 128 + tid] = U;
-      if (want_energy > 1) {
-        const int32_t K = wg_sum_planes<3, G::NW>(AK[a], red);
-        if (tid < 64) energy[(size_t)d.e_idx * 128 + 64 + tid] = K;
+      const uint64_t *opp = latW[a];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const uint64_t s = latB[a][tid + q * G::Q];
+        uint64_t nv[6];
+        nv[0] = opp[xi[q][0][0]];
+        nv[1] = opp[xi[q][0][1]];
+        nv[2] = opp[yz[q][0]];
+        nv[3] = opp[yz[q][1]];
+        nv[4] = opp[yz[q][2]];
+        nv[5] = opp[yz[q][3]];
+        uint64_t u0, u1, u2;
+        bond_count(s, nv, jw[q][0], u0, u1, u2);
+        vc_add3(AU[a], u0, u1, u2);
+        if (want_energy > 1) {
+          vc_add1(AK[a], s);
+          vc_add1(AK[a], latW[a][tid + q * G::Q]);
+        }
       }
     }
   }
@@ -504,6 +634,24 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
         g1[i] = latB[1][i];
         g1[G::NH + i] = latW[1][i];
       }
+  }
+
+  if (want_energy) {
+    int32_t U[2];
+    wg_count_lanes<4, G::NT, 2>(AU, fold, redi, U);
+    if (tid < 64) {
+      energy[(size_t)d0.e_idx * 128 + tid] = U[0];
+      if (two) energy[(size_t)d1.e_idx * 128 + tid] = U[1];
+    }
+    if (want_energy > 1) {
+      __syncthreads();  // fold and redi are reused
+      int32_t K[2];
+      wg_count_lanes<3, G::NT, 2>(AK, fold, redi, K);
+      if (tid < 64) {
+        energy[(size_t)d0.e_idx * 128 + 64 + tid] = K[0];
+        if (two) energy[(size_t)d1.e_idx * 128 + 64 + tid] = K[1];
+      }
+    }
   }
 }
 
