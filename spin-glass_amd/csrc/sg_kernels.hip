@@ -457,7 +457,6 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   __shared__ uint64_t latB[2][G::NH], latW[2][G::NH];
   __shared__ uint64_t fold[2 * 4 * G::NT];   // energy reduction scratch
   __shared__ int32_t redi[4 * 2 * 64];
-  __shared__ unsigned long long smask;
 
   const TaskDesc tk = ld_uniform(tasks + blockIdx.x);
   const JobPtrs jp = ld_uniform(jt + tk.job);
@@ -468,19 +467,40 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   const bool two = tk.lat[1] != kNone;
   const LatDesc d0 = ld_uniform(jp.descs + tk.lat[0]);
   const LatDesc d1 = ld_uniform(jp.descs + (two ? tk.lat[1] : tk.lat[0]));
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
   const bool active = tid < G::Q;
-  {
-    const SG_GLOBAL uint64_t *g0 = spins + d0.spin_off, *g1 = spins + d1.spin_off;
-    for (int i = tid; i < G::NH; i += G::NT) {
-      latB[0][i] = g0[i];
-      latW[0][i] = g0[G::NH + i];
-    }
-    if (two)
-      for (int i = tid; i < G::NH; i += G::NT) {
-        latB[1][i] = g1[i];
-        latW[1][i] = g1[G::NH + i];
+
+  // ---- PT step of the pair (t, t+1) at b = sweep / pt_every.  Every wave
+  // decides the 64 lanes (samples) itself, so the mask needs no LDS round
+  // trip; the energy loads go out first and the decision overlaps the
+  // lattice loads.
+  const bool pt_here = do_pt && two && tk.pair != kNone;
+  int32_t ua = 0, ub = 0, ka = 0, kb = 0;
+  PairDesc pd{};
+  uint32_t pt_b = 0;
+  if (pt_here) {
+    pd = ld_uniform(jp.pairs + tk.pair);
+    const JobState js = ld_uniform(jp.js);
+    pt_b = js.sweep / js.pt_every;
+    const SG_GLOBAL int32_t *Ea = energy + (size_t)pd.e_a * 128, *Eb = energy + (size_t)pd.e_b * 128;
+    ua = Ea[lane]; ub = Eb[lane]; ka = Ea[64 + lane]; kb = Eb[64 + lane];
+  }
+
+  // the task's lattices, at most two words per colour per thread (NH <= 2 NT)
+  static_assert(G::NH <= 2 * G::NT, "two load rounds");
+  const SG_GLOBAL uint64_t *g0 = spins + d0.spin_off, *g1 = spins + d1.spin_off;
+  uint64_t w[2][4];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int i = tid + k * G::NT;
+    if (i < G::NH) {
+      w[k][0] = g0[i];
+      w[k][1] = g0[G::NH + i];
+      if (two) {
+        w[k][2] = g1[i];
+        w[k][3] = g1[G::NH + i];
       }
+    }
   }
 
   // neighbour indices within the opposite colour's half (y, z: same for both
@@ -515,6 +535,31 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
       }
     }
   }
+
+  uint64_t m = 0;
+  if (pt_here) {
+    const uint32_t sw3 = (kTagPT << 28) | (pd.rt & 0x0FFFFFFFu);
+    const u32x4 r = philox_dev((uint32_t)(lane >> 2), pt_b, pd.g, sw3, pd.key0, pd.key1);
+    const bool acc = pt_accept(pd.beta_a, pd.beta_b, ua, ub, ka, kb, pd.H, r.v[lane & 3]);
+    m = __ballot(acc);
+  }
+  // accepted lanes trade their bits on the way into LDS
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int i = tid + k * G::NT;
+    if (i < G::NH) {
+      if (two) {
+        const uint64_t db = (w[k][0] ^ w[k][2]) & m, dw = (w[k][1] ^ w[k][3]) & m;
+        latB[0][i] = w[k][0] ^ db;
+        latW[0][i] = w[k][1] ^ dw;
+        latB[1][i] = w[k][2] ^ db;
+        latW[1][i] = w[k][3] ^ dw;
+      } else {
+        latB[0][i] = w[k][0];
+        latW[0][i] = w[k][1];
+      }
+    }
+  }
   ThrRec tr0{}, tr1{};
   uint32_t n0 = 0;
   if (nsweeps > 0) {
@@ -523,38 +568,6 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
     n0 = ld_uniform(jp.js).sweep;
   }
   __syncthreads();
-
-  // ---- PT exchange of the pair (t, t+1) at step b = sweep / pt_every
-  if (do_pt && two && tk.pair != kNone) {
-    const PairDesc p = ld_uniform(jp.pairs + tk.pair);
-    const JobState js = ld_uniform(jp.js);
-    if (tid < 64) {
-      const uint32_t b = js.sweep / js.pt_every;
-      const int l = tid;
-      const uint32_t sw3 = (kTagPT << 28) | (p.rt & 0x0FFFFFFFu);
-      const u32x4 r = philox_dev((uint32_t)(l >> 2), b, p.g, sw3, p.key0, p.key1);
-      const uint32_t rr = r.v[l & 3];
-      const SG_GLOBAL int32_t *Ea = energy + (size_t)p.e_a * 128, *Eb = energy + (size_t)p.e_b * 128;
-      const bool acc = pt_accept(p.beta_a, p.beta_b, Ea[l], Eb[l], Ea[64 + l], Eb[64 + l], p.H, rr);
-      const unsigned long long m = __ballot(acc);
-      if (l == 0) {
-        smask = m;
-        atomicAdd(&counters[p.cnt_idx], (unsigned long long)__popcll(m));
-      }
-    }
-    __syncthreads();
-    const uint64_t m = smask;
-    if (m)
-      for (int i = tid; i < G::NH; i += G::NT) {
-        uint64_t a = latB[0][i], bb = latB[1][i], dd = (a ^ bb) & m;
-        latB[0][i] = a ^ dd;
-        latB[1][i] = bb ^ dd;
-        a = latW[0][i]; bb = latW[1][i]; dd = (a ^ bb) & m;
-        latW[0][i] = a ^ dd;
-        latW[1][i] = bb ^ dd;
-      }
-    __syncthreads();
-  }
 
   const uint32_t sw0 = (kTagMC << 28) | (d0.rt & 0x0FFFFFFFu);
   const uint32_t sw1 = (kTagMC << 28) | (d1.rt & 0x0FFFFFFFu);
@@ -653,6 +666,7 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
       }
     }
   }
+  if (pt_here && tid == 0) atomicAdd(&counters[pd.cnt_idx], (unsigned long long)__popcll(m));
 }
 
 // ------------------------------------------------------- large-lattice kernel
