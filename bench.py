@@ -1,6 +1,7 @@
 """Spin-flip throughput of the parallel-tempering engine on one MI355X.
 
-Workload (per step = one PT interval of 16 sweeps over the whole batch):
+Default workload (--workload l16; per step = one PT interval of 16 sweeps
+over the whole batch):
   3D Edwards-Anderson +-J, L=16, 24 temperatures geometric in T = [0.2, 2.0],
   2 replicas per sample, PT attempt every 16 sweeps (alternating even/odd
   pairs), a measurement every 1000 sweeps, hot start from seeded Philox.
@@ -8,6 +9,17 @@ Workload (per step = one PT interval of 16 sweeps over the whole batch):
   launch, the interval captured as one launch sequence;
   plain form:   64 samples (one group), the per-group drop-in launches.
 value = batched spin-flip attempts per ns (whole job, inputs resident in HBM).
+
+Other workloads (one JSON line each, same keys):
+  mixed  8 groups of 64 samples, L per group drawn from {8,12,16,20,24}
+         (seed 0x5EED0003), 32 temperatures in [0.2, 2.0], 2 replicas, 1e5
+         sweeps each, all in one batch engine (one launch per L class and PT
+         interval) vs the same jobs launched one by one;
+  large  L = 32 and 48, 64 samples, 32 temperatures in [0.3, 2.0], 2 replicas;
+  serve  Poisson arrivals (200 jobs/s by default) of 64-sample jobs with
+         L ~ U{8,12,16,20,24}, sweeps log-uniform in [1e3, 1e5], 16..32
+         temperatures in [0.2, 2.0], own seeds; p50/p99 arrival-to-result
+         latency and throughput of the native serving loop.
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -36,6 +48,11 @@ W_ALG = 1.22          # int32 VALU ops per spin-flip attempt (algorithmic)
 # 16-lane-per-SIMD figure (39.3) understates the hardware by 2x.
 VALU_PEAK_TOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_block<16,false> launch of the l16 workload (768 pair tasks,
+# 16 sweeps): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of that kernel, each
+# scaled by what the counter reads for a device copy of known size (the
+# --copy-probe run); profiles/k_block_L16_batched_2048_traffic.txt.
+TRAFFIC_L16 = 131.8e6
 
 
 def make(samples, form):
@@ -86,6 +103,18 @@ def cpu_baseline(budget_s=10.0):
                        f"{dt:.1f} s")
 
 
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB (a known byte count for scaling
+    the FETCH_SIZE / WRITE_SIZE counters); outside every timed region."""
+    import torch
+    n = mib << 20
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,11 +122,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--workload", default="l16", choices=["l16", "mixed", "large", "serve"])
+    ap.add_argument("--sweeps", type=int, default=0, help="sweeps per job (mixed, large)")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
+    ap.add_argument("--jobs", type=int, default=400, help="serve: number of jobs")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (samples shard across GPUs with no "
-              "exchange; see DESIGN.md); use --gpus 1", file=sys.stderr)
+              "exchange); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+    if args.workload != "l16":
+        import bench_workloads
+        line = bench_workloads.run(args)
+        print(json.dumps(line), flush=True)
+        return
 
     b = time_form(SAMPLES_BATCHED, sg.SG_FORM_BATCHED, args.steps, args.warmup)
     value = b["flips"] / b["wall_s"] / 1e9
@@ -134,14 +175,14 @@ def main():
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "k_sweep_lds<16,false>",
+            "kernel": "k_block<16,false>",
             "achieved": round(achieved, 3),
             "peak": VALU_PEAK_TOPS,
             "unit": "TOP/s (int32 VALU, W_alg = 1.22 ops per flip attempt)",
             "frac": round(achieved / VALU_PEAK_TOPS, 4),
             "avg_launch_ms": round(avg_ms, 4),
             "flips_per_launch": flips_per_launch,
-            "traffic": None,
+            "traffic": TRAFFIC_L16,
         },
     }
     if not args.no_cpu:

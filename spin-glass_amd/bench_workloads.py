@@ -1,0 +1,183 @@
+"""Non-default bench.py workloads: mixed-L batch, large lattices, serving loop.
+
+Each returns one dict with bench.py's keys (metric, value, unit, n_gpus,
+steps, warmup, ms_per_step, higher_is_better, scaling, vs_baseline, dtype,
+data, config, roofline).  Inputs are synthetic: seeded Philox couplings and
+hot-start spins, as in the default workload.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+import sg
+
+PT_EVERY = 16
+VALU_PEAK_TOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+W_ALG = 1.22
+L_CHOICES = (8, 12, 16, 20, 24)
+
+
+def _base(metric, value, unit, wall_s, config, roofline, steps=0, warmup=0, higher=True,
+          scaling="strong"):
+    return {
+        "metric": metric, "value": round(value, 3), "unit": unit, "n_gpus": 1,
+        "steps": steps, "warmup": warmup,
+        "ms_per_step": round(wall_s * 1e3 / max(steps, 1), 4),
+        "higher_is_better": higher, "scaling": scaling, "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (seeded Philox couplings and hot-start spins)",
+        "config": config, "roofline": roofline,
+    }
+
+
+def _valu_roofline(flips_per_ns, kernel):
+    achieved = W_ALG * flips_per_ns * 1e9 / 1e12
+    return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3),
+            "peak": VALU_PEAK_TOPS,
+            "unit": "TOP/s (int32 VALU, W_alg = 1.22 ops per flip attempt; whole-job rate)",
+            "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None}
+
+
+def _run_batch(jobs, form, arena=8 << 30):
+    b = sg.Batch(pt_every=PT_EVERY, slice_pt_steps=4, form=form, arena_bytes=arena)
+    t0 = time.perf_counter()
+    ids = [b.submit(j) for j in jobs]
+    b.drain()
+    dt = time.perf_counter() - t0
+    for i in ids:
+        b.result(i)
+    b.close()
+    return dt
+
+
+# ------------------------------------------------------------------ mixed L
+def mixed(args):
+    rng = np.random.default_rng(0x5EED0003)
+    Ls = [int(x) for x in rng.choice(L_CHOICES, size=8)]
+    sweeps = args.sweeps or 100000
+    sweeps -= sweeps % PT_EVERY
+
+    def jobs(n):
+        return [sg.Job(L=L, n_sweeps=n, n_samples=64, T_range=(0.2, 2.0, 32),
+                       coupling_seed=0x5EED0001 + g, spin_seed=0x5EED1001 + g, meas_every=1024)
+                for g, L in enumerate(Ls)]
+
+    _run_batch(jobs(64), sg.SG_FORM_BATCHED)            # warm-up: code paths, arena
+    flips = sum(j.flips() for j in jobs(sweeps))
+    dt = _run_batch(jobs(sweeps), sg.SG_FORM_BATCHED)
+    value = flips / dt / 1e9
+    plain = None
+    if not args.no_plain:
+        ps = max(PT_EVERY, (sweeps // 10) - (sweeps // 10) % PT_EVERY)
+        pflips = sum(j.flips() for j in jobs(ps))
+        pdt = _run_batch(jobs(ps), sg.SG_FORM_PLAIN)
+        plain = pflips / pdt / 1e9
+    cfg = {"workload": f"8 groups x 64 samples, L per group {Ls} (drawn from {list(L_CHOICES)}, "
+                       f"seed 0x5EED0003), 32 temperatures geometric in T=[0.2,2.0], 2 replicas, "
+                       f"PT every {PT_EVERY}, measurement every 1024, {sweeps} sweeps per group",
+           "form": "batched engine: one launch per (L, field) class and PT interval, classes on "
+                   "separate streams",
+           "plain_flips_per_ns": None if plain is None else round(plain, 3),
+           "plain_workload": "same jobs, one launch per job and interval (sweeps / 10)",
+           "batched_over_plain": None if plain is None else round(value / plain, 3),
+           "parallelism": "1 GPU"}
+    return _base("spin flips/ns, mixed-L batch", value, "spin flips/ns", dt, cfg,
+                 _valu_roofline(value, "k_block<8|12|16>, k_big<20|24>"),
+                 steps=sweeps // PT_EVERY)
+
+
+# ------------------------------------------------------------------ large L
+def large(args):
+    sweeps = args.sweeps or 1024
+    sweeps -= sweeps % PT_EVERY
+    per_l = {}
+    tot_flips = tot_s = 0.0
+    for L in (32, 48):
+        s = sg.Sim(L=L, n_samples=64, n_replicas=2, T_range=(0.3, 2.0, 32), pt_every=PT_EVERY,
+                   meas_every=1024, form=sg.SG_FORM_BATCHED)
+        s.keep_records = False
+        s.init_lattice()
+        s.run(2 * PT_EVERY)
+        s.set_profiling(True)
+        t0 = time.perf_counter()
+        s.run(sweeps)
+        dt = time.perf_counter() - t0
+        n, ms, kfl = s.kernel_stats(0)
+        s.close()
+        flips = float(L) ** 3 * 64 * 32 * 2 * sweeps
+        fpn = flips / dt / 1e9
+        # algorithmic HBM bytes: each spin word read and written once per sweep
+        # (16 B per 64 flips), couplings 24 B per site per group and sweep
+        # shared by the 64 lattices of the group
+        bytes_per_flip = 16.0 / 64 + 24.0 / 64 / 64
+        gbs = fpn * bytes_per_flip
+        per_l[L] = {"flips_per_ns": round(fpn, 3), "kernel_ms_per_launch": round(ms / max(n, 1), 4),
+                    "hbm_alg_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        tot_flips += flips
+        tot_s += dt
+    value = tot_flips / tot_s / 1e9
+    cfg = {"workload": f"L=32 and L=48 (separately), 64 samples, 32 temperatures geometric in "
+                       f"T=[0.3,2.0], 2 replicas, PT every {PT_EVERY}, {sweeps} sweeps each",
+           "per_L": per_l, "form": "batched", "parallelism": "1 GPU"}
+    r = _valu_roofline(value, "k_big<32|48,false,false>")
+    r["hbm"] = {L: v["hbm_frac"] for L, v in per_l.items()}
+    return _base("spin flips/ns, L=32 and 48", value, "spin flips/ns", tot_s, cfg, r,
+                 steps=2 * sweeps // PT_EVERY)
+
+
+# ------------------------------------------------------------------ serving
+def serve_jobs(n, rate, seed=0x5EED0005):
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    arrivals -= arrivals[0]
+    jobs = []
+    for i in range(n):
+        L = int(rng.choice(L_CHOICES))
+        sw = float(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
+        sw = max(PT_EVERY, int(round(sw / PT_EVERY)) * PT_EVERY)
+        T = int(rng.integers(16, 33))
+        jobs.append(sg.Job(L=L, n_sweeps=sw, n_samples=64, T_range=(0.2, 2.0, T),
+                           coupling_seed=0x5EED2000 + i, spin_seed=0x5EED3000 + i,
+                           meas_every=1024))
+    return jobs, arrivals
+
+
+def serve(args):
+    jobs, arrivals = serve_jobs(args.jobs, args.rate)
+    b = sg.Batch(pt_every=PT_EVERY, slice_pt_steps=2, form=sg.SG_FORM_BATCHED,
+                 arena_bytes=16 << 30)
+    w = [b.submit(sg.Job(L=L, n_sweeps=64, T_range=(0.2, 2.0, 16))) for L in L_CHOICES]
+    b.drain()
+    for i in w:
+        b.result(i)
+    ids, wall = b.serve(jobs, arrivals)
+    lat = []
+    for i in ids:
+        t_sub, t_adm, t_done = b.times(int(i))
+        lat.append(t_done - t_sub)
+        b.result(int(i))
+    b.close()
+    lat = np.array(lat)
+    flips = sum(j.flips() for j in jobs)
+    offered = flips / arrivals[-1] / 1e9 if arrivals[-1] > 0 else float("nan")
+    value = flips / wall / 1e9
+    cfg = {"workload": f"{args.jobs} jobs, Poisson arrivals at {args.rate:g}/s (seed 0x5EED0005), "
+                       f"64 samples each, L ~ U{list(L_CHOICES)}, sweeps log-uniform in [1e3,1e5], "
+                       f"T ~ U{{16..32}} geometric in [0.2,2.0], 2 replicas, PT every {PT_EVERY}",
+           "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+           "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+           "mean_latency_s": round(float(lat.mean()), 4),
+           "offered_flips_per_ns": round(offered, 1),
+           "arrival_window_s": round(float(arrivals[-1]), 3),
+           "wall_s": round(wall, 3),
+           "form": "native serve loop: admission by footprint, continuous batching per L class",
+           "parallelism": "1 GPU"}
+    return _base("spin flips/ns and p50/p99 job latency, serving loop", value, "spin flips/ns",
+                 wall, cfg, _valu_roofline(value, "k_block<8|12|16>, k_big<20|24>"),
+                 steps=args.jobs)
+
+
+def run(args):
+    return {"mixed": mixed, "large": large, "serve": serve}[args.workload](args)
