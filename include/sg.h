@@ -60,11 +60,19 @@ typedef struct sg_params {
   int32_t meas_every;    /* sweeps between measurements (0 = none) */
   int32_t device;        /* HIP device ordinal */
   int32_t form;          /* SG_FORM_* */
+  int32_t kernel;        /* SG_KERNEL_*: how the lattices are held during sweeps */
+  int32_t pad;
 } sg_params;
 
 enum {
   SG_FORM_PLAIN = 0,      /* per-interval launches, one group at a time */
   SG_FORM_BATCHED = 1     /* all groups per launch, interval captured in a hipGraph */
+};
+
+enum {
+  SG_KERNEL_AUTO = 0,     /* chosen per L (and lattice count) from measured rates */
+  SG_KERNEL_RESIDENT = 1, /* lattice pair resident in one work-group for a PT interval */
+  SG_KERNEL_STREAMED = 2  /* one launch per colour, lattices streamed in chunks (L >= 12) */
 };
 
 /* Per-measurement callback: rec holds n_groups*n_temps*64 records of nfield

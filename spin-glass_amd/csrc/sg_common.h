@@ -146,6 +146,7 @@ struct JobState {
 struct JobPtrs {
   uint64_t *spins;
   const uint64_t *J;
+  const uint64_t *Js;  // coupling site records [group][colour][h][6]
   const LatDesc *descs;
   const PairDesc *pairs;
   const ThrRec *thr;

@@ -49,32 +49,45 @@ struct Job {
   ~Job() { free_host(); }
 };
 
-// Jobs of one (L, field) class: their stream, and the JobPtrs table and the
-// two task lists (each job at its own PT parity, and every parity flipped)
-// of the class's batched launch.
+// A device array with pinned staging, grown on demand.
+template <class T>
+struct DevList {
+  T *d = nullptr, *h = nullptr;
+  size_t cap = 0;
+  int n = 0;
+  void free_all() {
+    hipFree(d);
+    if (h) hipHostFree(h);
+    d = h = nullptr;
+    cap = 0;
+  }
+};
+
+// Jobs of one (L, field, resident|streamed) class: their stream, the JobPtrs
+// table and the task lists of the class's batched launches.  Every list
+// exists twice: [0] every job at its current PT parity, [1] every parity
+// flipped (the lists alternate from one PT interval to the next).
 struct Class {
   int L = 0;
-  bool field = false;
+  bool field = false, streamed = false;
   hipStream_t st = nullptr;
   std::vector<Job *> jobs;
   bool dirty = true;
   int64_t steps_since_build = 0;
-  JobPtrs *d_jt = nullptr;
-  TaskDesc *d_tasks[2] = {nullptr, nullptr};
-  size_t cap_jobs = 0, cap_tasks = 0;
-  JobPtrs *h_jt = nullptr;           // pinned staging
-  TaskDesc *h_tasks[2] = {nullptr, nullptr};
-  hipEvent_t up_ev = nullptr;        // the last upload from the staging
+  DevList<JobPtrs> jt;
+  DevList<TaskDesc> tasks[2];   // resident: pair/single tasks
+  DevList<TaskDesc> chunks;     // streamed: (lattice, chunk) tasks
+  DevList<TaskDesc> ptc[2];     // streamed: (pair, chunk) tasks of the PT launch
+  hipEvent_t up_ev = nullptr;   // the last upload from the staging
   bool uploaded = false;
-  int ntasks[2] = {0, 0};
 
   void free_all() {
-    hipFree(d_jt);
-    hipFree(d_tasks[0]);
-    hipFree(d_tasks[1]);
-    if (h_jt) hipHostFree(h_jt);
-    if (h_tasks[0]) hipHostFree(h_tasks[0]);
-    if (h_tasks[1]) hipHostFree(h_tasks[1]);
+    jt.free_all();
+    chunks.free_all();
+    for (int k = 0; k < 2; ++k) {
+      tasks[k].free_all();
+      ptc[k].free_all();
+    }
     if (up_ev) hipEventDestroy(up_ev);
     if (st) hipStreamDestroy(st);
   }
@@ -127,13 +140,14 @@ struct sg_batch {
     HIPCHK(arena.init((size_t)bp.arena_bytes));
   }
 
-  Class &class_for(int L, bool field) {
+  Class &class_for(int L, bool field, bool streamed) {
     for (auto &c : classes)
-      if (c.L == L && c.field == field) return c;
+      if (c.L == L && c.field == field && c.streamed == streamed) return c;
     classes.emplace_back();
     Class &c = classes.back();
     c.L = L;
     c.field = field;
+    c.streamed = streamed;
     HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c.up_ev, hipEventDisableTiming));
     return c;
@@ -165,12 +179,20 @@ struct sg_batch {
     j->p.pt_every = pe;
     j->p.device = bp.device;
     j->p.form = SG_FORM_BATCHED;
+    // engine jobs are single groups or few: resident only where one job's
+    // lattices fill the device well (L <= 16), unless the caller chose
+    if (in.kernel == SG_KERNEL_AUTO)
+      j->p.kernel = in.L <= 16 || !phase_kernel_supported(in.L) ? SG_KERNEL_RESIDENT
+                                                                 : SG_KERNEL_STREAMED;
+    else if (in.kernel == SG_KERNEL_STREAMED && !phase_kernel_supported(in.L))
+      throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
     if (in.betas) {
       j->betas.assign(in.betas, in.betas + in.n_temps);
       j->p.betas = j->betas.data();
     }
     j->n_sweeps = n_sweeps;
-    j->footprint = job_footprint(in.L, in.n_samples, in.n_replicas, in.n_temps);
+    j->footprint = job_footprint(in.L, in.n_samples, in.n_replicas, in.n_temps,
+                                 j->p.kernel == SG_KERNEL_STREAMED);
     if (j->footprint > arena.capacity())
       throw Fail{SG_ERR_NOMEM, "job footprint exceeds the arena"};
     j->t_submit = t_submit;
@@ -183,7 +205,7 @@ struct sg_batch {
   // ------------------------------------------------------------ admission
   bool try_admit(Job &j) {
     if (arena.capacity() - arena.in_use() < j.footprint) return false;
-    Class &c = class_for(j.p.L, j.p.H != 0.0);
+    Class &c = class_for(j.p.L, j.p.H != 0.0, j.p.kernel == SG_KERNEL_STREAMED);
     auto h = std::make_unique<sg_handle>();
     try {
       h->create(j.p, &arena, c.st);
@@ -250,53 +272,68 @@ struct sg_batch {
   }
 
   // ------------------------------------------------------------ a class's tables
-  void build(Class &c) {
+  // Grows a list so it holds n entries; queued launches may still read the
+  // old device array, so growth waits for the class's stream.
+  template <class T>
+  void reserve(Class &c, DevList<T> &l, size_t n) {
+    if (n <= l.cap && l.d) return;
+    HIPCHK(hipStreamSynchronize(c.st));
+    l.free_all();
+    l.cap = std::max<size_t>(2 * n, 64);
+    HIPCHK(hipMalloc((void **)&l.d, sizeof(T) * l.cap));
+    HIPCHK(hipHostMalloc((void **)&l.h, sizeof(T) * l.cap));
+  }
+
+  template <class T>
+  void upload(Class &c, DevList<T> &l) {
+    if (l.n > 0)
+      HIPCHK(hipMemcpyAsync(l.d, l.h, sizeof(T) * l.n, hipMemcpyHostToDevice, c.st));
+  }
+
+  // One job's tasks appended with their job index; with pairs_first, the
+  // pair tasks of all jobs go first, so the dispatcher's greedy fill ends
+  // evenly.
+  void gather(Class &c, DevList<TaskDesc> &out, bool pairs_first,
+              const std::vector<TaskDesc> &(*pick)(sg_handle &, int), int k) {
     size_t total = 0;
-    for (Job *j : c.jobs)
-      total += std::max(j->h->h_tasks[0].size(), j->h->h_tasks[1].size());
+    for (Job *j : c.jobs) total += pick(*j->h, (int)(((j->h->sweep / bp.pt_every) + k) & 1)).size();
+    reserve(c, out, total);
+    out.n = 0;
+    for (int pass = 0; pass < (pairs_first ? 2 : 1); ++pass)
+      for (size_t ji = 0; ji < c.jobs.size(); ++ji) {
+        sg_handle &h = *c.jobs[ji]->h;
+        const int par = (int)(((h.sweep / bp.pt_every) + k) & 1);
+        for (TaskDesc t : pick(h, par)) {
+          if (pairs_first && ((t.lat[1] != 0xFFFFFFFFu) != (pass == 0))) continue;
+          t.job = (uint32_t)ji;
+          out.h[out.n++] = t;
+        }
+      }
+  }
+
+  static const std::vector<TaskDesc> &pick_tasks(sg_handle &h, int par) { return h.h_tasks[par]; }
+  static const std::vector<TaskDesc> &pick_chunks(sg_handle &h, int) { return h.h_chunks; }
+  static const std::vector<TaskDesc> &pick_ptc(sg_handle &h, int par) { return h.h_ptc[par]; }
+
+  void build(Class &c) {
     if (c.uploaded) HIPCHK(hipEventSynchronize(c.up_ev));  // staging free again
-    if (c.jobs.size() > c.cap_jobs || total > c.cap_tasks) {
-      // grow: queued launches may still read the old tables, so wait for them
-      HIPCHK(hipStreamSynchronize(c.st));
-      hipFree(c.d_jt); hipFree(c.d_tasks[0]); hipFree(c.d_tasks[1]);
-      if (c.h_jt) hipHostFree(c.h_jt);
-      if (c.h_tasks[0]) hipHostFree(c.h_tasks[0]);
-      if (c.h_tasks[1]) hipHostFree(c.h_tasks[1]);
-      c.d_jt = nullptr; c.d_tasks[0] = c.d_tasks[1] = nullptr;
-      c.h_jt = nullptr; c.h_tasks[0] = c.h_tasks[1] = nullptr;
-      c.cap_jobs = std::max<size_t>(2 * c.jobs.size(), 16);
-      c.cap_tasks = std::max<size_t>(2 * total, 1024);
-      HIPCHK(hipMalloc((void **)&c.d_jt, sizeof(JobPtrs) * c.cap_jobs));
-      HIPCHK(hipHostMalloc((void **)&c.h_jt, sizeof(JobPtrs) * c.cap_jobs));
+    reserve(c, c.jt, c.jobs.size());
+    c.jt.n = (int)c.jobs.size();
+    for (size_t ji = 0; ji < c.jobs.size(); ++ji) c.jt.h[ji] = c.jobs[ji]->h->self;
+    upload(c, c.jt);
+    if (c.streamed) {
+      gather(c, c.chunks, false, pick_chunks, 0);
+      upload(c, c.chunks);
       for (int k = 0; k < 2; ++k) {
-        HIPCHK(hipMalloc((void **)&c.d_tasks[k], sizeof(TaskDesc) * c.cap_tasks));
-        HIPCHK(hipHostMalloc((void **)&c.h_tasks[k], sizeof(TaskDesc) * c.cap_tasks));
+        gather(c, c.ptc[k], false, pick_ptc, k);
+        upload(c, c.ptc[k]);
+      }
+    } else {
+      for (int k = 0; k < 2; ++k) {
+        gather(c, c.tasks[k], true, pick_tasks, k);
+        upload(c, c.tasks[k]);
       }
     }
-    // list 0: every job at its current PT parity; list 1: every parity
-    // flipped.  Pair tasks of all jobs first, so the dispatcher's greedy
-    // fill ends evenly.
-    for (int k = 0; k < 2; ++k) {
-      size_t n = 0;
-      for (int pass = 0; pass < 2; ++pass)
-        for (size_t ji = 0; ji < c.jobs.size(); ++ji) {
-          sg_handle &h = *c.jobs[ji]->h;
-          const int par = (int)(((h.sweep / bp.pt_every) + k) & 1);
-          for (TaskDesc t : h.h_tasks[par]) {
-            const bool pair = t.lat[1] != 0xFFFFFFFFu;
-            if (pair != (pass == 0)) continue;
-            t.job = (uint32_t)ji;
-            c.h_tasks[k][n++] = t;
-          }
-        }
-      c.ntasks[k] = (int)n;
-    }
-    for (size_t ji = 0; ji < c.jobs.size(); ++ji) c.h_jt[ji] = c.jobs[ji]->h->self;
-    HIPCHK(hipMemcpyAsync(c.d_jt, c.h_jt, sizeof(JobPtrs) * c.jobs.size(),
-                          hipMemcpyHostToDevice, c.st));
-    for (int k = 0; k < 2; ++k)
-      HIPCHK(hipMemcpyAsync(c.d_tasks[k], c.h_tasks[k], sizeof(TaskDesc) * c.ntasks[k],
-                            hipMemcpyHostToDevice, c.st));
     HIPCHK(hipEventRecord(c.up_ev, c.st));
     c.uploaded = true;
     c.dirty = false;
@@ -331,6 +368,8 @@ struct sg_batch {
     }
   }
 
+  // one PT interval of every job of the class: PT, pt_every sweeps, the
+  // energies of the final states, the sweep counters
   void run_class(Class &c) {
     const int pe = bp.pt_every;
     for (int i = 0; i < bp.slice_pt_steps && !c.jobs.empty(); ++i) {
@@ -338,16 +377,27 @@ struct sg_batch {
       if (bp.form == SG_FORM_BATCHED) {
         if (c.dirty) build(c);
         const int k = (int)(c.steps_since_build & 1);
-        HIPCHK(launch_block(c.L, c.field, c.d_jt, c.d_tasks[k], c.ntasks[k], 1, pe, emode, c.st));
-        HIPCHK(launch_advance(c.d_jt, (int)c.jobs.size(), pe, c.st));
+        if (c.streamed) {
+          const int pm = kPhEnergy | (emode > 1 ? kPhEnergyK : 0);
+          HIPCHK(launch_pt_chunks(c.jt.d, c.ptc[k].d, c.ptc[k].n, c.st));
+          for (int it = 0; it < pe; ++it)
+            for (int col = 0; col < 2; ++col) {
+              int mode = kPhUpdate;
+              if (it == 0 && col == 0) mode |= kPhZero;
+              if (it == pe - 1 && col == 1) mode |= pm;
+              HIPCHK(launch_phase(c.L, c.field, c.jt.d, c.chunks.d, c.chunks.n, col, it, mode,
+                                  c.st));
+            }
+        } else {
+          HIPCHK(launch_block(c.L, c.field, c.jt.d, c.tasks[k].d, c.tasks[k].n, 1, pe, emode,
+                              c.st));
+        }
+        HIPCHK(launch_advance(c.jt.d, c.jt.n, pe, c.st));
         ++c.steps_since_build;
       } else {
         for (Job *jp : c.jobs) {
           sg_handle &h = *jp->h;
-          const int par = (int)((h.sweep / pe) & 1);
-          HIPCHK(launch_block(c.L, c.field, h.d_self, h.d_tasks[par],
-                              (int)h.h_tasks[par].size(), 1, pe, emode, c.st));
-          HIPCHK(launch_advance(h.d_self, 1, pe, c.st));
+          h.launch_interval((int)((h.sweep / pe) & 1), 1, pe, emode);
         }
       }
       after_interval(c);
@@ -475,7 +525,9 @@ const char *sg_batch_last_error(const sg_batch *b) {
 
 int64_t sg_job_footprint(const sg_params *p) {
   if (!p || p->L < 2 || p->n_samples < 1 || p->n_temps < 1 || p->n_replicas < 1) return -1;
-  return (int64_t)job_footprint(p->L, p->n_samples, p->n_replicas, p->n_temps);
+  const bool streamed = p->kernel == SG_KERNEL_STREAMED ||
+                        (p->kernel == SG_KERNEL_AUTO && p->L > 16 && phase_kernel_supported(p->L));
+  return (int64_t)job_footprint(p->L, p->n_samples, p->n_replicas, p->n_temps, streamed);
 }
 
 sg_status sg_submit(sg_batch *b, const sg_params *p, int64_t n_sweeps, int64_t *id) {

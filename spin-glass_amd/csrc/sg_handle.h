@@ -33,6 +33,47 @@ namespace detail {
 
 inline thread_local std::string g_create_error;
 
+constexpr uint32_t kNoPair = 0xFFFFFFFFu;
+
+// The dispatcher hands work-group i of a launch to XCD i % 8, and each XCD
+// has its own L2.  Streamed launches therefore order their (lattice, chunk)
+// tasks so that every chunk of a lattice lands on one XCD, the same one in
+// every launch: a lattice written by one colour launch is read by the next
+// from that XCD's L2.  Lattice k of the list (in order of appearance) takes
+// slot k % 8; the slots interleave, short ones padded with no-op tasks
+// (chunk index past the last quad: no spins, no energy slot touched).
+constexpr int kXcds = 8;
+
+inline std::vector<TaskDesc> xcd_order(const std::vector<TaskDesc> &in, uint32_t pad_chunk) {
+  std::vector<std::vector<TaskDesc>> slot(kXcds);
+  int k = -1;
+  uint32_t last_lat = 0xFFFFFFFFu, last_job = 0xFFFFFFFFu;
+  for (const TaskDesc &t : in) {
+    if (t.lat[0] != last_lat || t.job != last_job) {
+      ++k;
+      last_lat = t.lat[0];
+      last_job = t.job;
+    }
+    slot[k % kXcds].push_back(t);
+  }
+  size_t m = 0;
+  for (auto &v : slot) m = std::max(m, v.size());
+  std::vector<TaskDesc> out;
+  if (in.empty()) return out;
+  out.reserve(m * kXcds);
+  for (size_t i = 0; i < m; ++i)
+    for (int x = 0; x < kXcds; ++x) {
+      if (i < slot[x].size()) {
+        out.push_back(slot[x][i]);
+      } else {
+        TaskDesc t = in.front();
+        t.lat[1] = pad_chunk;
+        out.push_back(t);
+      }
+    }
+  return out;
+}
+
 struct Fail {
   sg_status code;
   std::string msg;
@@ -54,19 +95,33 @@ inline uint64_t fnv1a(const void *data, size_t n, uint64_t h = 14695981039346656
   return h;
 }
 
+// Resident (lattice pair in one work-group for a PT interval) or streamed
+// (one launch per colour over chunks of every lattice) sweeps, for
+// SG_KERNEL_AUTO: streaming wins where a work-group cannot hold the pair in
+// LDS and the couplings in VGPRs, or where too few lattices leave CUs idle.
+inline bool auto_streamed(int L, size_t n_lattices) {
+  if (L <= 16 || !phase_kernel_supported(L)) return false;
+  if (L >= 22) return true;
+  return n_lattices < 1024;
+}
+
 // Device bytes one engine job needs, block by block as sg_handle::create
 // allocates them from an arena (each block rounded to the arena's
 // alignment; no record buffer).
-inline size_t job_footprint(int L, int n_samples, int R, int T) {
+inline size_t job_footprint(int L, int n_samples, int R, int T, bool streamed) {
   const size_t N = (size_t)L * L * L, G = (size_t)(n_samples + 63) / 64;
   const size_t nlat = G * R * T;
   const size_t npairs = G * R * (size_t)(T > 1 ? T - 1 : 0);
   auto r = [](size_t b) { return DevArena::round_up(b ? b : 1); };
-  return r(8 * nlat * N) + r(8 * G * 3 * N) + r(sizeof(ThrRec) * T) + r(4 * nlat * 128) +
-         r(sizeof(LatDesc) * nlat) + r(sizeof(MeasDesc) * G * T) +
-         r(sizeof(JobState)) + r(8 * (size_t)(T > 1 ? T - 1 : 1)) + r(sizeof(PairDesc) * npairs) +
-         2 * r(sizeof(TaskDesc) * nlat) + r(sizeof(ObsAcc) * G * T * 64) + r(16 * (size_t)L) +
-         r(sizeof(JobPtrs));
+  size_t b = r(8 * nlat * N) + r(8 * G * 3 * N) + r(8 * G * 6 * N) + r(sizeof(ThrRec) * T) +
+             r(4 * nlat * 128) + r(sizeof(LatDesc) * nlat) + r(sizeof(MeasDesc) * G * T) +
+             r(sizeof(JobState)) + r(8 * (size_t)(T > 1 ? T - 1 : 1)) +
+             r(sizeof(PairDesc) * npairs) + 2 * r(sizeof(TaskDesc) * nlat) +
+             r(sizeof(ObsAcc) * G * T * 64) + r(16 * (size_t)L) + r(sizeof(JobPtrs));
+  if (streamed)
+    b += r(sizeof(TaskDesc) * nlat * phase_chunks(L)) +
+         2 * r(sizeof(TaskDesc) * (npairs / 2 + G * R) * pt_chunks(L));
+  return b;
 }
 
 }  // namespace detail
@@ -121,6 +176,15 @@ struct sg_handle {
   JobPtrs self{};
   JobPtrs *d_self = nullptr;
   std::vector<TaskDesc> h_tasks[2];
+
+  // streamed sweeps: (lattice, chunk) tasks of the colour launches and
+  // (pair, chunk) tasks of the PT launches of parity 0 and 1, ordered by
+  // group (chunk_off / ptc_off)
+  bool streamed = false;
+  uint64_t *d_Js = nullptr;  // coupling site records [group][colour][h][6]
+  TaskDesc *d_chunks = nullptr, *d_ptc[2] = {nullptr, nullptr};
+  std::vector<TaskDesc> h_chunks, h_ptc[2];
+  std::vector<int> chunk_off, ptc_off[2];
 
   uint64_t *d_spins = nullptr, *d_J = nullptr;
   ThrRec *d_thr = nullptr;
@@ -210,7 +274,7 @@ struct sg_handle {
     dev_free(d_spins); dev_free(d_J); dev_free(d_thr); dev_free(d_energy); dev_free(d_rec);
     dev_free(d_lat); dev_free(d_pairs); dev_free(d_meas); dev_free(d_jobs); dev_free(d_cnt);
     dev_free(d_acc); dev_free(d_cs); dev_free(d_tasks[0]); dev_free(d_tasks[1]);
-    dev_free(d_self);
+    dev_free(d_self); dev_free(d_Js); dev_free(d_chunks); dev_free(d_ptc[0]); dev_free(d_ptc[1]);
     for (auto &g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();
     for (int i = 0; i < kRing; ++i) {
@@ -265,8 +329,15 @@ struct sg_handle {
     HIPCHK(hipEventCreate(&ev1));
     nfield = 2 * R + 1 + 3 * L;
 
+    if (p.kernel == SG_KERNEL_STREAMED) {
+      if (!phase_kernel_supported(L)) throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
+      streamed = true;
+    } else if (p.kernel == SG_KERNEL_AUTO) {
+      streamed = auto_streamed(L, n_lat());
+    }
     dev_alloc(&d_spins, n_lat() * N);
     dev_alloc(&d_J, (size_t)G * 3 * N);
+    dev_alloc(&d_Js, (size_t)G * 6 * N);
     dev_alloc(&d_thr, (size_t)T);
     dev_alloc(&d_energy, n_lat() * 128);
     // engine jobs (arena) never deliver records: no buffer for them
@@ -384,10 +455,58 @@ struct sg_handle {
     swap_att.assign((size_t)std::max(T - 1, 1), 0);
     reset_counters();
 
-    self.spins = d_spins; self.J = d_J; self.descs = d_lat; self.pairs = d_pairs;
+    if (streamed) build_stream_tasks();
+    self.spins = d_spins; self.J = d_J; self.Js = d_Js; self.descs = d_lat; self.pairs = d_pairs;
     self.thr = d_thr; self.js = d_jobs; self.energy = d_energy; self.counters = d_cnt;
     dev_alloc(&d_self, 1);
     HIPCHK(hipMemcpy(d_self, &self, sizeof(JobPtrs), hipMemcpyHostToDevice));
+  }
+
+  void build_stream_tasks() {
+    const int nc = phase_chunks(L), np = pt_chunks(L);
+    h_chunks.clear();
+    chunk_off.assign(G + 1, 0);
+    const bool per_group = form == SG_FORM_PLAIN;
+    std::vector<TaskDesc> run;
+    for (int g = 0; g < G; ++g) {
+      if (per_group) chunk_off[g] = (int)h_chunks.size();
+      for (int rho = 0; rho < R; ++rho)
+        for (int t = 0; t < T; ++t)
+          for (int k = 0; k < nc; ++k)
+            run.push_back(TaskDesc{{(uint32_t)lat_index(g, rho, t), (uint32_t)k}, kNoPair, 0});
+      if (per_group || g == G - 1) {
+        const std::vector<TaskDesc> o = xcd_order(run, (uint32_t)nc);
+        h_chunks.insert(h_chunks.end(), o.begin(), o.end());
+        run.clear();
+      }
+    }
+    chunk_off[G] = (int)h_chunks.size();
+    dev_alloc(&d_chunks, h_chunks.size());
+    HIPCHK(hipMemcpy(d_chunks, h_chunks.data(), sizeof(TaskDesc) * h_chunks.size(),
+                     hipMemcpyHostToDevice));
+    for (int par = 0; par < 2; ++par) {
+      h_ptc[par].clear();
+      ptc_off[par].assign(G + 1, 0);
+      for (int g = 0; g < G; ++g) {
+        ptc_off[par][g] = (int)h_ptc[par].size();
+        for (int rho = 0; rho < R; ++rho)
+          for (int t = par; t + 1 < T; t += 2)
+            for (int k = 0; k < np; ++k)
+              h_ptc[par].push_back(TaskDesc{
+                  {(uint32_t)(((size_t)g * R + rho) * (T - 1) + t), (uint32_t)k}, kNoPair, 0});
+      }
+      ptc_off[par][G] = (int)h_ptc[par].size();
+      dev_alloc(&d_ptc[par], h_ptc[par].size());
+      if (!h_ptc[par].empty())
+        HIPCHK(hipMemcpy(d_ptc[par], h_ptc[par].data(), sizeof(TaskDesc) * h_ptc[par].size(),
+                         hipMemcpyHostToDevice));
+    }
+  }
+
+  // coupling site records from the [3][N] couplings (after every change of J)
+  void build_jsite() {
+    for (int g = 0; g < G; ++g)
+      HIPCHK(launch_build_jsite(L, d_Js, d_J, (uint64_t)g * 3 * N, st));
   }
 
   void ensure_ring() {
@@ -436,6 +555,7 @@ struct sg_handle {
     for (int g = 0; g < G; ++g)
       HIPCHK(launch_init_couplings(L, d_J, (uint64_t)g * 3 * N, (uint32_t)g, (uint32_t)cseed,
                                    (uint32_t)(cseed >> 32), st));
+    build_jsite();
     HIPCHK(launch_init_spins(L, d_spins, d_lat, (int)n_lat(), st));
     sweep = 0;
     n_meas = 0;
@@ -481,6 +601,10 @@ struct sg_handle {
   // batched form, one launch per group in the plain form), then the device
   // sweep counter advance.  do_pt: the launch starts at a PT step.
   void launch_interval(int par, int do_pt, int seg, int want_energy) {
+    if (streamed) {
+      launch_interval_streamed(par, do_pt, seg, want_energy);
+      return;
+    }
     const bool per_group = form == SG_FORM_PLAIN;
     for (int g = 0; g < (per_group ? G : 1); ++g) {
       const int a = per_group ? task_off[par][g] : 0;
@@ -495,7 +619,43 @@ struct sg_handle {
   }
 
   void launch_energy_all(int mode) {
+    if (streamed) {
+      launch_interval_streamed(0, 0, 0, mode);
+      return;
+    }
     HIPCHK(launch_block(L, field, d_self, d_tasks[0], task_off[0][G], 0, 0, mode, st));
+  }
+
+  // Streamed interval: the PT launch, then one launch per colour and sweep;
+  // the first zeroes the energy slots, the last accumulates the energies of
+  // the final states.  seg = 0 with want_energy: energies only.
+  void launch_interval_streamed(int par, int do_pt, int seg, int want_energy) {
+    const bool per_group = form == SG_FORM_PLAIN;
+    const int emode = want_energy ? (kPhEnergy | (want_energy > 1 ? kPhEnergyK : 0)) : 0;
+    for (int g = 0; g < (per_group ? G : 1); ++g) {
+      const int ca = per_group ? chunk_off[g] : 0, cb = per_group ? chunk_off[g + 1] : chunk_off[G];
+      const int pa = per_group ? ptc_off[par][g] : 0;
+      const int pb = per_group ? ptc_off[par][g + 1] : ptc_off[par][G];
+      if (do_pt) HIPCHK(launch_pt_chunks(d_self, d_ptc[par] + pa, pb - pa, st));
+      if (seg == 0) {
+        if (want_energy) {
+          HIPCHK(launch_phase(L, field, d_self, d_chunks + ca, cb - ca, 0, 0, kPhZero, st));
+          HIPCHK(launch_phase(L, field, d_self, d_chunks + ca, cb - ca, 1, 0, emode, st));
+        }
+        continue;
+      }
+      const double fl = (double)seg * N * 64.0 * R * T * (per_group ? 1 : G);
+      timed(0, fl, [&] {
+        for (int it = 0; it < seg; ++it)
+          for (int c = 0; c < 2; ++c) {
+            int mode = kPhUpdate;
+            if (it == 0 && c == 0 && want_energy) mode |= kPhZero;
+            if (it == seg - 1 && c == 1) mode |= emode;
+            HIPCHK(launch_phase(L, field, d_self, d_chunks + ca, cb - ca, c, it, mode, st));
+          }
+      });
+    }
+    if (seg > 0) HIPCHK(launch_advance(d_self, 1, seg, st));
   }
 
   // Batched form: a full PT interval is one hipGraph per (parity, length,
@@ -720,6 +880,7 @@ struct sg_handle {
     initialized = true;
     set_spins((const uint64_t *)p); p += sizeof(uint64_t) * n_lat() * N;
     HIPCHK(hipMemcpy(d_J, p, sizeof(uint64_t) * (size_t)G * 3 * N, hipMemcpyHostToDevice));
+    build_jsite();
     p += sizeof(uint64_t) * (size_t)G * 3 * N;
     const size_t nc = std::max(T - 1, 1);
     HIPCHK(hipMemcpy(d_cnt, p, sizeof(uint64_t) * nc, hipMemcpyHostToDevice)); p += sizeof(uint64_t) * nc;

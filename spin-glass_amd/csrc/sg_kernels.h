@@ -29,6 +29,26 @@ hipError_t launch_init_spins(int L, uint64_t *spins, const LatDesc *descs,
 hipError_t launch_init_couplings(int L, uint64_t *J, uint64_t j_off, uint32_t g,
                                  uint32_t key0, uint32_t key1, hipStream_t st);
 
+// ---- streamed lattices: one launch per colour (k_phase), PT in its own launch
+constexpr int kPhaseNT = 256;     // threads per work-group = Philox quads per chunk
+constexpr int kPtWords = 1024;    // words of each lattice per PT work-group
+enum : int { kPhUpdate = 1, kPhEnergy = 2, kPhEnergyK = 4, kPhZero = 8 };
+
+inline int phase_chunks(int L) { return (L * L * L / 4 + kPhaseNT - 1) / kPhaseNT; }
+inline int pt_chunks(int L) { return (L * L * L + kPtWords - 1) / kPtWords; }
+bool phase_kernel_supported(int L);
+
+// Tasks: lat[0] = lattice descriptor, lat[1] = chunk, job.  c = colour, it =
+// sweep within the interval (the Philox counter is the job's sweep + it),
+// mode = kPh* bits (energy bits with c = 1 only).
+hipError_t launch_phase(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
+                        int c, int it, int mode, hipStream_t st);
+// Tasks: lat[0] = PairDesc index, lat[1] = chunk, job.
+hipError_t launch_pt_chunks(const JobPtrs *jt, const TaskDesc *tasks, int ntasks, hipStream_t st);
+// coupling site records of one group from its [3][N] couplings
+hipError_t launch_build_jsite(int L, uint64_t *Js, const uint64_t *J, uint64_t j_off,
+                              hipStream_t st);
+
 // adds nsweeps to the sweep counter of each of the njobs jobs of jt
 hipError_t launch_advance(const JobPtrs *jt, int njobs, int nsweeps, hipStream_t st);
 

@@ -1040,6 +1040,152 @@ __global__ void k_advance(const JobPtrs *jt, int njobs, int nsweeps) {
   if (i < njobs) jt[i].js->sweep += (uint32_t)nsweeps;
 }
 
+// ------------------------------------------------------- coupling site records
+// Site record of colour c, checkerboard index h: the six bond words of that
+// site (+x, -x, +y, -y, +z, -z) at word (c*NH + h)*6 of the group's block
+// (6N words, at 2*j_off).  One site's couplings are then three 16-byte loads
+// and a wave reads 3 KiB contiguous, where the natural [3][N] layout gives
+// six 8-byte loads at stride 2.
+template <int L>
+__global__ __launch_bounds__(256) void k_build_jsite(uint64_t *__restrict__ Js,
+                                                     const uint64_t *__restrict__ J,
+                                                     uint64_t j_off) {
+  constexpr int N = L * L * L, NH = N / 2, LH = L / 2;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  const int c = i / NH, h = i % NH;
+  const int xh = h % LH, y = (h / LH) % L, z = h / (LH * L);
+  const int x = 2 * xh + ((y + z + c) & 1);
+  const int xm = (x + L - 1) % L, ym = (y + L - 1) % L, zm = (z + L - 1) % L;
+  const uint64_t *Jx = J + j_off, *Jy = Jx + N, *Jz = Jy + N;
+  const int s = x + L * (y + L * z);
+  uint64_t *o = Js + 2 * j_off + (uint64_t)i * 6;
+  o[0] = Jx[s];
+  o[1] = Jx[xm + L * (y + L * z)];
+  o[2] = Jy[s];
+  o[3] = Jy[x + L * (ym + L * z)];
+  o[4] = Jz[s];
+  o[5] = Jz[x + L * (y + L * zm)];
+}
+
+// ---------------------------------------------------------- per-phase kernel
+// One launch updates one colour of every lattice of its task list, the
+// lattices streamed from HBM/L2 in chunks of kPhaseNT Philox quads: a task
+// is (lattice, chunk), a thread owns quad j and updates the sites h = j and
+// h = j + N/4 of the launch's colour (words 2c and 2c+1 of the quad's Philox
+// block).  The kernel boundary orders the phases, so a lattice spreads over
+// as many work-groups as it has chunks (108 at L = 48) with no waiting
+// between them.  mode bits: update the spins; zero the lattice's energy
+// slots (chunk 0; before any accumulation, in an earlier launch); accumulate
+// U (and K) of the states after this launch (colour 1 only: every bond has
+// one white end, every site index h one white and one black word) into the
+// slots with per-lane atomics.
+template <int L, bool FIELD>
+__global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ jt,
+                                                    const TaskDesc *__restrict__ tasks, int c,
+                                                    int it, int mode) {
+  using G = Geo<L>;
+  __shared__ uint64_t fold[4 * kPhaseNT];
+  __shared__ int32_t redi[4 * 64];
+  const TaskDesc tk = ld_uniform(tasks + blockIdx.x);
+  const JobPtrs jp = ld_uniform(jt + tk.job);
+  const LatDesc d = ld_uniform(jp.descs + tk.lat[0]);
+  SG_GLOBAL int32_t *__restrict__ energy = to_global(jp.energy) + (size_t)d.e_idx * 128;
+  const int tid = threadIdx.x;
+  if ((mode & kPhZero) && tk.lat[1] == 0 && tid < 128) energy[tid] = 0;
+  const int j = (int)tk.lat[1] * kPhaseNT + tid;
+  const bool active = j < G::Q;
+  SG_GLOBAL uint64_t *__restrict__ lat = to_global(jp.spins) + d.spin_off;
+  SG_GLOBAL uint64_t *__restrict__ mine = lat + c * G::NH;
+  const SG_GLOBAL uint64_t *__restrict__ opp = lat + (1 - c) * G::NH;
+  const SG_GLOBAL uint64_t *__restrict__ Js =
+      to_global(jp.Js) + 2 * d.j_off + (uint64_t)c * G::NH * 6;
+  const bool upd = (mode & kPhUpdate) != 0;
+  const bool eng = (mode & (kPhEnergy | kPhEnergyK)) != 0;
+  u32x4 r{};
+  ThrRec tr{};
+  if (upd) {
+    tr = ld_uniform(jp.thr + d.thr_idx);
+    const uint32_t n = ld_uniform(jp.js).sweep + (uint32_t)it;
+    r = philox_dev((uint32_t)j, n, d.g, (kTagMC << 28) | (d.rt & 0x0FFFFFFFu), d.key0, d.key1);
+  }
+  uint64_t AU[1][4] = {{0, 0, 0, 0}}, AK[1][3] = {{0, 0, 0}};
+  if (active && (upd || eng)) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int h = j + q * G::Q;
+      int nb[6], s_nat, jn[6];
+      site_geometry<L>(h, c, nb, s_nat, jn);
+      const SG_GLOBAL uint64_t *jr = Js + (size_t)h * 6;
+      uint64_t jw[6], nv[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        jw[k] = jr[k];
+        nv[k] = opp[nb[k]];
+      }
+      uint64_t s = mine[h];
+      if (upd) {
+        update_site<L, FIELD>(s, nv, jw, r.v[2 * c + q], tr);
+        mine[h] = s;
+      }
+      if (eng) {
+        uint64_t u0, u1, u2;
+        bond_count(s, nv, jw, u0, u1, u2);
+        vc_add3(AU[0], u0, u1, u2);
+        if (mode & kPhEnergyK) {
+          vc_add1(AK[0], s);
+          vc_add1(AK[0], opp[h]);
+        }
+      }
+    }
+  }
+  if (eng) {
+    int32_t U[1];
+    wg_count_lanes<4, kPhaseNT, 1>(AU, fold, redi, U);
+    if (tid < 64) atomicAdd((int32_t *)(energy + tid), U[0]);
+    if (mode & kPhEnergyK) {
+      __syncthreads();
+      int32_t K[1];
+      wg_count_lanes<3, kPhaseNT, 1>(AK, fold, redi, K);
+      if (tid < 64) atomicAdd((int32_t *)(energy + 64 + tid), K[0]);
+    }
+  }
+}
+
+// PT step of streamed lattices: a task is (pair, chunk of kPtWords words);
+// every work-group decides the 64 lanes itself (same energies, same Philox
+// words, so the same mask) and exchanges its chunk; chunk 0 counts.
+__global__ __launch_bounds__(256) void k_pt_chunk(const JobPtrs *__restrict__ jt,
+                                                  const TaskDesc *__restrict__ tasks) {
+  const TaskDesc tk = ld_uniform(tasks + blockIdx.x);
+  const JobPtrs jp = ld_uniform(jt + tk.job);
+  const PairDesc p = ld_uniform(jp.pairs + tk.lat[0]);
+  const JobState js = ld_uniform(jp.js);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const uint32_t b = js.sweep / js.pt_every;
+  const SG_GLOBAL int32_t *E = to_global(jp.energy);
+  const SG_GLOBAL int32_t *Ea = E + (size_t)p.e_a * 128, *Eb = E + (size_t)p.e_b * 128;
+  const u32x4 r = philox_dev((uint32_t)(lane >> 2), b, p.g, (kTagPT << 28) | (p.rt & 0x0FFFFFFFu),
+                             p.key0, p.key1);
+  const bool acc = pt_accept(p.beta_a, p.beta_b, Ea[lane], Eb[lane], Ea[64 + lane], Eb[64 + lane],
+                             p.H, r.v[lane & 3]);
+  const uint64_t m = __ballot(acc);
+  if (m) {
+    SG_GLOBAL uint64_t *A = to_global(jp.spins) + p.a_off, *B = to_global(jp.spins) + p.b_off;
+    const uint32_t i0 = tk.lat[1] * kPtWords;
+#pragma unroll
+    for (int k = 0; k < kPtWords / 256; ++k) {
+      const uint32_t i = i0 + (uint32_t)(k * 256 + tid);
+      if (i < p.n_words) {
+        const uint64_t a = A[i], bb = B[i], dd = (a ^ bb) & m;
+        A[i] = a ^ dd;
+        B[i] = bb ^ dd;
+      }
+    }
+  }
+  if (tk.lat[1] == 0 && tid == 0) atomicAdd(&jp.counters[p.cnt_idx], (unsigned long long)__popcll(m));
+}
+
 // ---------------------------------------------------------------- dispatch
 #define SG_LDS_SIZES(X) X(2) X(4) X(6) X(8) X(10) X(12) X(14) X(16)
 #define SG_ALL_SIZES(X) SG_LDS_SIZES(X) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) \
@@ -1128,6 +1274,62 @@ hipError_t launch_init_couplings(int L, uint64_t *J, uint64_t j_off, uint32_t g,
   hipLaunchKernelGGL(k_init_couplings, dim3((N + 255) / 256, 3), dim3(256), 0, st, J, j_off,
                      N, g, key0, key1);
   return hipGetLastError();
+}
+
+#define SG_PHASE_SIZES(X) X(12) X(14) X(16) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) \
+  X(34) X(36) X(38) X(40) X(42) X(44) X(46) X(48)
+
+bool phase_kernel_supported(int L) {
+  switch (L) {
+#define X(l) case l:
+    SG_PHASE_SIZES(X)
+#undef X
+    return true;
+    default:
+      return false;
+  }
+}
+
+hipError_t launch_phase(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
+                        int c, int it, int mode, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  switch (L) {
+#define X(l)                                                                               \
+  case l:                                                                                  \
+    if (field)                                                                             \
+      hipLaunchKernelGGL((k_phase<l, true>), dim3(ntasks), dim3(kPhaseNT), 0, st, jt, tasks, \
+                         c, it, mode);                                                     \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_phase<l, false>), dim3(ntasks), dim3(kPhaseNT), 0, st, jt,      \
+                         tasks, c, it, mode);                                              \
+    return hipGetLastError();
+    SG_PHASE_SIZES(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_pt_chunks(const JobPtrs *jt, const TaskDesc *tasks, int ntasks, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pt_chunk, dim3(ntasks), dim3(256), 0, st, jt, tasks);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_jsite(int L, uint64_t *Js, const uint64_t *J, uint64_t j_off,
+                              hipStream_t st) {
+  const int N = L * L * L;
+  switch (L) {
+#define X(l)                                                                               \
+  case l:                                                                                  \
+    hipLaunchKernelGGL(k_build_jsite<l>, dim3((N + 255) / 256), dim3(256), 0, st, Js, J,   \
+                       j_off);                                                             \
+    return hipGetLastError();
+    SG_ALL_SIZES(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_advance(const JobPtrs *jt, int njobs, int nsweeps, hipStream_t st) {

@@ -48,6 +48,9 @@ sg_status sg_create(const sg_params *p, sg_handle **out) {
   if (p->pt_every < 0 || p->meas_every < 0) return bad("negative cadence");
   if (!p->betas && !(p->T_min > 0.0 && p->T_max > 0.0)) return bad("need betas or T_min/T_max > 0");
   if (p->form != SG_FORM_PLAIN && p->form != SG_FORM_BATCHED) return bad("unknown form");
+  if (p->kernel < SG_KERNEL_AUTO || p->kernel > SG_KERNEL_STREAMED) return bad("unknown kernel");
+  if (p->kernel == SG_KERNEL_STREAMED && !phase_kernel_supported(p->L))
+    return bad("no streamed kernel for this L");
   sg_handle *h = new (std::nothrow) sg_handle();
   if (!h) return bad("host allocation failed");
   try {
@@ -127,6 +130,7 @@ sg_status sg_set_couplings(sg_handle *h, const uint64_t *in) {
     HIPCHK(hipStreamSynchronize(h->st));
     HIPCHK(hipMemcpy(h->d_J, in, sizeof(uint64_t) * (size_t)h->G * 3 * h->N,
                      hipMemcpyHostToDevice));
+    h->build_jsite();
     h->energy_valid = false;
   });
 }

@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libsg.so")
 
 SG_FORM_PLAIN = 0
 SG_FORM_BATCHED = 1
+SG_KERNEL_AUTO, SG_KERNEL_RESIDENT, SG_KERNEL_STREAMED = 0, 1, 2
 RES_COLUMNS = ["beta", "e", "m", "q", "q2", "q4", "binder", "chi0", "chi1", "xi_L", "n_meas"]
 NRES = len(RES_COLUMNS)
 
@@ -40,6 +41,7 @@ class _Params(ctypes.Structure):
         ("coupling_seed", ctypes.c_uint64), ("spin_seed", ctypes.c_uint64),
         ("pt_every", ctypes.c_int32), ("meas_every", ctypes.c_int32),
         ("device", ctypes.c_int32), ("form", ctypes.c_int32),
+        ("kernel", ctypes.c_int32), ("pad", ctypes.c_int32),
     ]
 
 
@@ -133,7 +135,7 @@ def _check(status, h=None):
 
 
 def _fill_params(p, L, n_samples, n_replicas, betas, T_range, H, coupling_seed, spin_seed,
-                 pt_every, meas_every, device=0, form=SG_FORM_PLAIN):
+                 pt_every, meas_every, device=0, form=SG_FORM_PLAIN, kernel=SG_KERNEL_AUTO):
     """Fills an sg_params; returns the betas array that p.betas points into
     (keep it alive while p is used)."""
     p.L, p.n_samples, p.n_replicas = L, n_samples, n_replicas
@@ -150,6 +152,7 @@ def _fill_params(p, L, n_samples, n_replicas, betas, T_range, H, coupling_seed, 
     p.H = H
     p.coupling_seed, p.spin_seed = coupling_seed, spin_seed
     p.pt_every, p.meas_every, p.device, p.form = pt_every, meas_every, device, form
+    p.kernel = kernel
     return keep
 
 
@@ -158,11 +161,12 @@ class Sim:
 
     def __init__(self, L=16, n_samples=64, n_replicas=2, betas=None, T_range=None, H=0.0,
                  coupling_seed=0x5EED0001, spin_seed=0x5EED0002, pt_every=16,
-                 meas_every=0, device=0, form=SG_FORM_PLAIN):
+                 meas_every=0, device=0, form=SG_FORM_PLAIN, kernel=SG_KERNEL_AUTO):
         lib = load_library()
         p = _Params()
         self._betas = _fill_params(p, L, n_samples, n_replicas, betas, T_range, H,
-                                   coupling_seed, spin_seed, pt_every, meas_every, device, form)
+                                   coupling_seed, spin_seed, pt_every, meas_every, device, form,
+                                   kernel)
         h = _vp()
         _check(lib.sg_create(ctypes.byref(p), ctypes.byref(h)), None)
         self._h = h
@@ -288,8 +292,10 @@ class Job:
     """Parameters of one engine job (the arguments of Sim, plus n_sweeps)."""
 
     def __init__(self, L, n_sweeps, n_samples=64, n_replicas=2, betas=None, T_range=None,
-                 H=0.0, coupling_seed=0x5EED0001, spin_seed=0x5EED0002, meas_every=0):
+                 H=0.0, coupling_seed=0x5EED0001, spin_seed=0x5EED0002, meas_every=0,
+                 kernel=SG_KERNEL_AUTO):
         self.L, self.n_sweeps, self.n_samples, self.n_replicas = L, n_sweeps, n_samples, n_replicas
+        self.kernel = kernel
         self.betas, self.T_range, self.H = betas, T_range, H
         self.coupling_seed, self.spin_seed, self.meas_every = coupling_seed, spin_seed, meas_every
 
@@ -300,7 +306,8 @@ class Job:
     def params(self, pt_every=0):
         p = _Params()
         keep = _fill_params(p, self.L, self.n_samples, self.n_replicas, self.betas, self.T_range,
-                            self.H, self.coupling_seed, self.spin_seed, pt_every, self.meas_every)
+                            self.H, self.coupling_seed, self.spin_seed, pt_every, self.meas_every,
+                            kernel=self.kernel)
         return p, keep
 
     def footprint(self):

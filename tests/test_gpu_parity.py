@@ -15,10 +15,10 @@ CSEED, SSEED = 0x5EED0001, 0x5EED0002
 
 
 def _pair(L, n_samples, R, betas, H=0.0, pt_every=1, meas_every=0, form=sg.SG_FORM_PLAIN,
-          cseed=CSEED, sseed=SSEED):
+          cseed=CSEED, sseed=SSEED, kernel=sg.SG_KERNEL_AUTO):
     sim = sg.Sim(L=L, n_samples=n_samples, n_replicas=R, betas=betas, H=H,
                  coupling_seed=cseed, spin_seed=sseed, pt_every=pt_every,
-                 meas_every=meas_every, form=form)
+                 meas_every=meas_every, form=form, kernel=kernel)
     sim.init_lattice()
     orc = oracle.OracleSim(L, sim.G, R, betas, H=H, coupling_seed=cseed, spin_seed=sseed,
                            pt_every=pt_every, meas_every=meas_every)
@@ -70,7 +70,7 @@ def test_large_lattice_sweeps_match_oracle(L):
     updated in place in global memory above; 80 samples = one full and one
     partial lane group; 4 temperatures give pair-only and mixed task lists."""
     betas = geometric_betas(0.3, 2.0, 4)
-    sim, orc = _pair(L, 80, 2, betas, pt_every=2, meas_every=3)
+    sim, orc = _pair(L, 80, 2, betas, pt_every=2, meas_every=3, kernel=sg.SG_KERNEL_RESIDENT)
     sim.run(4)
     orc.run(4)
     _assert_same(sim, orc)
@@ -82,10 +82,56 @@ def test_large_lattice_sweeps_match_oracle(L):
         np.testing.assert_array_equal(rec, orc.meas(i))
 
 
+@pytest.mark.parametrize("L", [12, 16, 18, 24, 32, 48])
+def test_streamed_kernel_matches_oracle(L):
+    """One launch per colour over (lattice, chunk) tasks, PT in its own
+    launch, energies accumulated by the last colour launch."""
+    betas = geometric_betas(0.3, 2.0, 4)
+    sim, orc = _pair(L, 80, 2, betas, pt_every=2, meas_every=3, kernel=sg.SG_KERNEL_STREAMED)
+    sim.run(4)
+    orc.run(4)
+    _assert_same(sim, orc)
+    sim.run(3)
+    orc.run(3)
+    _assert_same(sim, orc)
+    for i, (_, rec) in enumerate(sim.records):
+        np.testing.assert_array_equal(rec, orc.meas(i))
+
+
+@pytest.mark.parametrize("H", [0.25, 1.5])
+def test_streamed_kernel_field_matches_oracle(H):
+    betas = geometric_betas(0.3, 2.0, 3)
+    sim, orc = _pair(20, 64, 2, betas, H=H, pt_every=1, meas_every=3,
+                     kernel=sg.SG_KERNEL_STREAMED)
+    sim.run(5)
+    orc.run(5)
+    _assert_same(sim, orc)
+    np.testing.assert_array_equal(sim.records[0][1], orc.meas(0))
+
+
+def test_resident_and_streamed_kernels_agree_in_both_forms():
+    betas = geometric_betas(0.2, 2.0, 6)
+    runs = []
+    for kernel in (sg.SG_KERNEL_RESIDENT, sg.SG_KERNEL_STREAMED):
+        for form in (sg.SG_FORM_PLAIN, sg.SG_FORM_BATCHED):
+            s = sg.Sim(L=20, n_samples=128, n_replicas=2, betas=betas, pt_every=4, meas_every=8,
+                       form=form, kernel=kernel)
+            s.init_lattice()
+            s.run(36)          # the batched form replays its captured intervals
+            runs.append((s.spins(), s.energies(), s.swap_counts(), s.results()))
+    for r in runs[1:]:
+        np.testing.assert_array_equal(r[0], runs[0][0])
+        np.testing.assert_array_equal(r[1][0], runs[0][1][0])
+        np.testing.assert_array_equal(r[1][1], runs[0][1][1])
+        np.testing.assert_array_equal(r[2][0], runs[0][2][0])
+        np.testing.assert_array_equal(r[3], runs[0][3])
+
+
 @pytest.mark.parametrize("L", [20, 26])
 def test_large_lattice_field_matches_oracle(L):
     betas = geometric_betas(0.3, 2.0, 3)
-    sim, orc = _pair(L, 64, 2, betas, H=0.5, pt_every=1, meas_every=3)
+    sim, orc = _pair(L, 64, 2, betas, H=0.5, pt_every=1, meas_every=3,
+                     kernel=sg.SG_KERNEL_RESIDENT)
     sim.run(5)
     orc.run(5)
     _assert_same(sim, orc)
