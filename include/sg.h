@@ -151,6 +151,14 @@ sg_status sg_get_timing(sg_handle *h, double *kernel_ms, double *flips);
 sg_status sg_set_profiling(sg_handle *h, int32_t on);
 sg_status sg_get_kernel_stats(sg_handle *h, int32_t kind, int64_t *launches,
                               double *total_ms, double *flips);
+/* the autotuner behind SG_KERNEL_AUTO (no counterpart in the reference): the
+ * kernel it picks for n_lattices lattices of size L on a device, with the
+ * measured device time of both candidates over the same 64 sweeps (0 when
+ * only the resident kernel exists for this L).  The first call for a
+ * (device, L, field, power-of-two lattice count) measures; later ones are
+ * answered from the process-wide cache.  Errors: sg_last_error(NULL). */
+sg_status sg_tune_kernel(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
+                         int32_t *kernel, double *resident_ms, double *streamed_ms);
 
 /* --- batch and serving engine -------------------------------------------
  * No counterpart in the reference, which runs one fixed-size simulation per

@@ -63,12 +63,15 @@ struct DevList {
   }
 };
 
-// Jobs of one (L, field, resident|streamed) class: their stream, the JobPtrs
+// Jobs of one (L, field, requested kernel) class: their stream, the JobPtrs
 // table and the task lists of the class's batched launches.  Every list
 // exists twice: [0] every job at its current PT parity, [1] every parity
-// flipped (the lists alternate from one PT interval to the next).
+// flipped (the lists alternate from one PT interval to the next).  With
+// SG_KERNEL_AUTO the class picks resident or streamed sweeps whenever its
+// membership changes, from the autotuner's measurement for its lattice count.
 struct Class {
   int L = 0;
+  int kernel = SG_KERNEL_AUTO;
   bool field = false, streamed = false;
   hipStream_t st = nullptr;
   std::vector<Job *> jobs;
@@ -140,14 +143,14 @@ struct sg_batch {
     HIPCHK(arena.init((size_t)bp.arena_bytes));
   }
 
-  Class &class_for(int L, bool field, bool streamed) {
+  Class &class_for(int L, bool field, int kernel) {
     for (auto &c : classes)
-      if (c.L == L && c.field == field && c.streamed == streamed) return c;
+      if (c.L == L && c.field == field && c.kernel == kernel) return c;
     classes.emplace_back();
     Class &c = classes.back();
     c.L = L;
     c.field = field;
-    c.streamed = streamed;
+    c.kernel = kernel;
     HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c.up_ev, hipEventDisableTiming));
     return c;
@@ -179,20 +182,16 @@ struct sg_batch {
     j->p.pt_every = pe;
     j->p.device = bp.device;
     j->p.form = SG_FORM_BATCHED;
-    // engine jobs are single groups or few: resident only where one job's
-    // lattices fill the device well (L <= 16), unless the caller chose
-    if (in.kernel == SG_KERNEL_AUTO)
-      j->p.kernel = in.L <= 16 || !phase_kernel_supported(in.L) ? SG_KERNEL_RESIDENT
-                                                                 : SG_KERNEL_STREAMED;
-    else if (in.kernel == SG_KERNEL_STREAMED && !phase_kernel_supported(in.L))
+    if (in.kernel < SG_KERNEL_AUTO || in.kernel > SG_KERNEL_STREAMED)
+      throw Fail{SG_ERR_INVALID, "unknown kernel"};
+    if (in.kernel == SG_KERNEL_STREAMED && !phase_kernel_supported(in.L))
       throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
     if (in.betas) {
       j->betas.assign(in.betas, in.betas + in.n_temps);
       j->p.betas = j->betas.data();
     }
     j->n_sweeps = n_sweeps;
-    j->footprint = job_footprint(in.L, in.n_samples, in.n_replicas, in.n_temps,
-                                 j->p.kernel == SG_KERNEL_STREAMED);
+    j->footprint = job_footprint(in.L, in.n_samples, in.n_replicas, in.n_temps);
     if (j->footprint > arena.capacity())
       throw Fail{SG_ERR_NOMEM, "job footprint exceeds the arena"};
     j->t_submit = t_submit;
@@ -205,7 +204,7 @@ struct sg_batch {
   // ------------------------------------------------------------ admission
   bool try_admit(Job &j) {
     if (arena.capacity() - arena.in_use() < j.footprint) return false;
-    Class &c = class_for(j.p.L, j.p.H != 0.0, j.p.kernel == SG_KERNEL_STREAMED);
+    Class &c = class_for(j.p.L, j.p.H != 0.0, j.p.kernel);
     auto h = std::make_unique<sg_handle>();
     try {
       h->create(j.p, &arena, c.st);
@@ -317,6 +316,16 @@ struct sg_batch {
 
   void build(Class &c) {
     if (c.uploaded) HIPCHK(hipEventSynchronize(c.up_ev));  // staging free again
+    if (c.kernel == SG_KERNEL_STREAMED) {
+      c.streamed = true;
+    } else if (c.kernel == SG_KERNEL_RESIDENT || !phase_kernel_supported(c.L)) {
+      c.streamed = false;
+    } else {
+      size_t nlat = 0;
+      for (Job *j : c.jobs) nlat += j->h->n_lat();
+      c.streamed = tune_streamed(bp.device, c.L, c.field, nlat);
+    }
+    for (Job *j : c.jobs) j->h->set_streamed(c.streamed);
     reserve(c, c.jt, c.jobs.size());
     c.jt.n = (int)c.jobs.size();
     for (size_t ji = 0; ji < c.jobs.size(); ++ji) c.jt.h[ji] = c.jobs[ji]->h->self;
@@ -525,9 +534,7 @@ const char *sg_batch_last_error(const sg_batch *b) {
 
 int64_t sg_job_footprint(const sg_params *p) {
   if (!p || p->L < 2 || p->n_samples < 1 || p->n_temps < 1 || p->n_replicas < 1) return -1;
-  const bool streamed = p->kernel == SG_KERNEL_STREAMED ||
-                        (p->kernel == SG_KERNEL_AUTO && p->L > 16 && phase_kernel_supported(p->L));
-  return (int64_t)job_footprint(p->L, p->n_samples, p->n_replicas, p->n_temps, streamed);
+  return (int64_t)job_footprint(p->L, p->n_samples, p->n_replicas, p->n_temps);
 }
 
 sg_status sg_submit(sg_batch *b, const sg_params *p, int64_t n_sweeps, int64_t *id) {

@@ -95,20 +95,17 @@ inline uint64_t fnv1a(const void *data, size_t n, uint64_t h = 14695981039346656
   return h;
 }
 
-// Resident (lattice pair in one work-group for a PT interval) or streamed
-// (one launch per colour over chunks of every lattice) sweeps, for
-// SG_KERNEL_AUTO: streaming wins where a work-group cannot hold the pair in
-// LDS and the couplings in VGPRs, or where too few lattices leave CUs idle.
-inline bool auto_streamed(int L, size_t n_lattices) {
-  if (L <= 16 || !phase_kernel_supported(L)) return false;
-  if (L >= 22) return true;
-  return n_lattices < 1024;
-}
+// Autotuner (sg_runtime.cpp): resident (lattice pair in one work-group for
+// a PT interval) or streamed (one launch per colour over chunks of every
+// lattice) sweeps for n_lattices lattices of size L, decided by timing both
+// on the device; cached per (device, L, field, power-of-two lattice count).
+bool tune_streamed(int device, int L, bool field, size_t n_lattices);
 
 // Device bytes one engine job needs, block by block as sg_handle::create
 // allocates them from an arena (each block rounded to the arena's
 // alignment; no record buffer).
-inline size_t job_footprint(int L, int n_samples, int R, int T, bool streamed) {
+inline size_t job_footprint(int L, int n_samples, int R, int T) {
+  const bool streamed = phase_kernel_supported(L);  // both task sets are kept
   const size_t N = (size_t)L * L * L, G = (size_t)(n_samples + 63) / 64;
   const size_t nlat = G * R * T;
   const size_t npairs = G * R * (size_t)(T > 1 ? T - 1 : 0);
@@ -119,7 +116,8 @@ inline size_t job_footprint(int L, int n_samples, int R, int T, bool streamed) {
              r(sizeof(PairDesc) * npairs) + 2 * r(sizeof(TaskDesc) * nlat) +
              r(sizeof(ObsAcc) * G * T * 64) + r(16 * (size_t)L) + r(sizeof(JobPtrs));
   if (streamed)
-    b += r(sizeof(TaskDesc) * nlat * phase_chunks(L)) +
+    // chunk tasks: XCD padding adds at most 7 lattices' worth per group
+    b += r(8 * nlat * N / 4) + r(sizeof(TaskDesc) * (nlat + 7 * G) * phase_chunks(L)) +
          2 * r(sizeof(TaskDesc) * (npairs / 2 + G * R) * pt_chunks(L));
   return b;
 }
@@ -179,9 +177,12 @@ struct sg_handle {
 
   // streamed sweeps: (lattice, chunk) tasks of the colour launches and
   // (pair, chunk) tasks of the PT launches of parity 0 and 1, ordered by
-  // group (chunk_off / ptc_off)
-  bool streamed = false;
+  // group (chunk_off / ptc_off).  Both kernels use the same data, so a
+  // handle that can stream keeps both task sets and `streamed` may change
+  // between PT intervals.
+  bool can_stream = false, streamed = false;
   uint64_t *d_Js = nullptr;  // coupling site records [group][colour][h][6]
+  uint64_t *d_rng = nullptr; // streamed: stashed Philox words [lattice][N/4]
   TaskDesc *d_chunks = nullptr, *d_ptc[2] = {nullptr, nullptr};
   std::vector<TaskDesc> h_chunks, h_ptc[2];
   std::vector<int> chunk_off, ptc_off[2];
@@ -274,7 +275,7 @@ struct sg_handle {
     dev_free(d_spins); dev_free(d_J); dev_free(d_thr); dev_free(d_energy); dev_free(d_rec);
     dev_free(d_lat); dev_free(d_pairs); dev_free(d_meas); dev_free(d_jobs); dev_free(d_cnt);
     dev_free(d_acc); dev_free(d_cs); dev_free(d_tasks[0]); dev_free(d_tasks[1]);
-    dev_free(d_self); dev_free(d_Js); dev_free(d_chunks); dev_free(d_ptc[0]); dev_free(d_ptc[1]);
+    dev_free(d_self); dev_free(d_Js); dev_free(d_rng); dev_free(d_chunks); dev_free(d_ptc[0]); dev_free(d_ptc[1]);
     for (auto &g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();
     for (int i = 0; i < kRing; ++i) {
@@ -329,12 +330,9 @@ struct sg_handle {
     HIPCHK(hipEventCreate(&ev1));
     nfield = 2 * R + 1 + 3 * L;
 
-    if (p.kernel == SG_KERNEL_STREAMED) {
-      if (!phase_kernel_supported(L)) throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
-      streamed = true;
-    } else if (p.kernel == SG_KERNEL_AUTO) {
-      streamed = auto_streamed(L, n_lat());
-    }
+    can_stream = phase_kernel_supported(L);
+    if (p.kernel == SG_KERNEL_STREAMED && !can_stream)
+      throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
     dev_alloc(&d_spins, n_lat() * N);
     dev_alloc(&d_J, (size_t)G * 3 * N);
     dev_alloc(&d_Js, (size_t)G * 6 * N);
@@ -455,11 +453,29 @@ struct sg_handle {
     swap_att.assign((size_t)std::max(T - 1, 1), 0);
     reset_counters();
 
-    if (streamed) build_stream_tasks();
-    self.spins = d_spins; self.J = d_J; self.Js = d_Js; self.descs = d_lat; self.pairs = d_pairs;
+    if (can_stream) {
+      build_stream_tasks();
+      dev_alloc(&d_rng, n_lat() * (size_t)(N / 4));
+    }
+    self.spins = d_spins; self.J = d_J; self.Js = d_Js; self.rng = d_rng;
+    self.descs = d_lat; self.pairs = d_pairs;
     self.thr = d_thr; self.js = d_jobs; self.energy = d_energy; self.counters = d_cnt;
     dev_alloc(&d_self, 1);
     HIPCHK(hipMemcpy(d_self, &self, sizeof(JobPtrs), hipMemcpyHostToDevice));
+    if (p.kernel == SG_KERNEL_STREAMED)
+      streamed = true;
+    else if (p.kernel == SG_KERNEL_AUTO && can_stream && !arena)
+      streamed = tune_streamed(device, L, field, n_lat());
+  }
+
+  // switches between the kernels (between PT intervals); captured graphs
+  // hold the old launch sequence
+  void set_streamed(bool on) {
+    if (on == streamed) return;
+    if (on && !can_stream) throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
+    streamed = on;
+    for (auto &g : graphs) hipGraphExecDestroy(g.exec);
+    graphs.clear();
   }
 
   void build_stream_tasks() {

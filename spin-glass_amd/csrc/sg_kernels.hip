@@ -1102,39 +1102,77 @@ __global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ 
       to_global(jp.Js) + 2 * d.j_off + (uint64_t)c * G::NH * 6;
   const bool upd = (mode & kPhUpdate) != 0;
   const bool eng = (mode & (kPhEnergy | kPhEnergyK)) != 0;
-  u32x4 r{};
+  // quad j = xh + (L/2)(y + L z) with z < L/2; its sites are (xh, y, z) and
+  // (xh, y, z + L/2): one column, one set of divisions.  Every load is
+  // issued before the Philox rounds, which then hide its latency.
+  const int xh = j % G::LH, y = (j / G::LH) % L, z0 = j / (G::LH * L);
+  const int ry = G::LH * y;
+  const int ryp = G::LH * (y + 1 == L ? 0 : y + 1), rym = G::LH * (y == 0 ? L - 1 : y - 1);
+  constexpr int PL = G::LH * L;  // words per plane of one colour
+  const bool work = active && (upd || eng);
+  int hq[2];
+  uint64_t jw[2][6], nv[2][6], sq[2], bq[2] = {0, 0};
+  if (work) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int z = z0 + q * (L / 2);
+      const int pz = PL * z;
+      const int pzp = PL * (z + 1 == L ? 0 : z + 1), pzm = PL * (z == 0 ? L - 1 : z - 1);
+      const int x = 2 * xh + ((y + z + c) & 1);
+      const int xp = x + 1 == L ? 0 : x + 1, xm = x == 0 ? L - 1 : x - 1;
+      const int h = xh + ry + pz;
+      hq[q] = h;
+      const SG_GLOBAL uint64_t *jr = Js + (size_t)h * 6;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) jw[q][k] = jr[k];
+      nv[q][0] = opp[(xp >> 1) + ry + pz];
+      nv[q][1] = opp[(xm >> 1) + ry + pz];
+      nv[q][2] = opp[xh + ryp + pz];
+      nv[q][3] = opp[xh + rym + pz];
+      nv[q][4] = opp[xh + ry + pzp];
+      nv[q][5] = opp[xh + ry + pzm];
+      sq[q] = mine[h];
+      if (mode & kPhEnergyK) bq[q] = opp[h];
+    }
+  }
+  // the quad's Philox block: colour 0 computes it, uses words 0 and 1 and
+  // stashes words 2 and 3 for the colour-1 launch of the same sweep
+  SG_GLOBAL uint64_t *__restrict__ stash = to_global(jp.rng) + d.spin_off / 4;
+  uint32_t rw[2] = {0, 0};
   ThrRec tr{};
   if (upd) {
     tr = ld_uniform(jp.thr + d.thr_idx);
-    const uint32_t n = ld_uniform(jp.js).sweep + (uint32_t)it;
-    r = philox_dev((uint32_t)j, n, d.g, (kTagMC << 28) | (d.rt & 0x0FFFFFFFu), d.key0, d.key1);
+    if (active) {
+      if (c == 0) {
+        const uint32_t n = ld_uniform(jp.js).sweep + (uint32_t)it;
+        const u32x4 r =
+            philox_dev((uint32_t)j, n, d.g, (kTagMC << 28) | (d.rt & 0x0FFFFFFFu), d.key0, d.key1);
+        rw[0] = r.v[0];
+        rw[1] = r.v[1];
+        stash[j] = (uint64_t)r.v[2] | ((uint64_t)r.v[3] << 32);
+      } else {
+        const uint64_t w = stash[j];
+        rw[0] = (uint32_t)w;
+        rw[1] = (uint32_t)(w >> 32);
+      }
+    }
   }
   uint64_t AU[1][4] = {{0, 0, 0, 0}}, AK[1][3] = {{0, 0, 0}};
-  if (active && (upd || eng)) {
+  if (work) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      const int h = j + q * G::Q;
-      int nb[6], s_nat, jn[6];
-      site_geometry<L>(h, c, nb, s_nat, jn);
-      const SG_GLOBAL uint64_t *jr = Js + (size_t)h * 6;
-      uint64_t jw[6], nv[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        jw[k] = jr[k];
-        nv[k] = opp[nb[k]];
-      }
-      uint64_t s = mine[h];
+      uint64_t s = sq[q];
       if (upd) {
-        update_site<L, FIELD>(s, nv, jw, r.v[2 * c + q], tr);
-        mine[h] = s;
+        update_site<L, FIELD>(s, nv[q], jw[q], rw[q], tr);
+        mine[hq[q]] = s;
       }
       if (eng) {
         uint64_t u0, u1, u2;
-        bond_count(s, nv, jw, u0, u1, u2);
+        bond_count(s, nv[q], jw[q], u0, u1, u2);
         vc_add3(AU[0], u0, u1, u2);
         if (mode & kPhEnergyK) {
           vc_add1(AK[0], s);
-          vc_add1(AK[0], opp[h]);
+          vc_add1(AK[0], bq[q]);
         }
       }
     }
@@ -1162,6 +1200,16 @@ __global__ __launch_bounds__(256) void k_pt_chunk(const JobPtrs *__restrict__ jt
   const PairDesc p = ld_uniform(jp.pairs + tk.lat[0]);
   const JobState js = ld_uniform(jp.js);
   const int tid = threadIdx.x, lane = tid & 63;
+  // the chunk's words go out before the decision
+  SG_GLOBAL uint64_t *A = to_global(jp.spins) + p.a_off, *B = to_global(jp.spins) + p.b_off;
+  const uint32_t i0 = tk.lat[1] * kPtWords;
+  uint64_t wa[kPtWords / 256], wb[kPtWords / 256];
+#pragma unroll
+  for (int k = 0; k < kPtWords / 256; ++k) {
+    const uint32_t i = i0 + (uint32_t)(k * 256 + tid);
+    wa[k] = i < p.n_words ? A[i] : 0;
+    wb[k] = i < p.n_words ? B[i] : 0;
+  }
   const uint32_t b = js.sweep / js.pt_every;
   const SG_GLOBAL int32_t *E = to_global(jp.energy);
   const SG_GLOBAL int32_t *Ea = E + (size_t)p.e_a * 128, *Eb = E + (size_t)p.e_b * 128;
@@ -1171,15 +1219,13 @@ __global__ __launch_bounds__(256) void k_pt_chunk(const JobPtrs *__restrict__ jt
                              p.H, r.v[lane & 3]);
   const uint64_t m = __ballot(acc);
   if (m) {
-    SG_GLOBAL uint64_t *A = to_global(jp.spins) + p.a_off, *B = to_global(jp.spins) + p.b_off;
-    const uint32_t i0 = tk.lat[1] * kPtWords;
 #pragma unroll
     for (int k = 0; k < kPtWords / 256; ++k) {
       const uint32_t i = i0 + (uint32_t)(k * 256 + tid);
       if (i < p.n_words) {
-        const uint64_t a = A[i], bb = B[i], dd = (a ^ bb) & m;
-        A[i] = a ^ dd;
-        B[i] = bb ^ dd;
+        const uint64_t dd = (wa[k] ^ wb[k]) & m;
+        A[i] = wa[k] ^ dd;
+        B[i] = wb[k] ^ dd;
       }
     }
   }

@@ -1,8 +1,66 @@
 // sg_runtime.cpp — C-ABI of libsg (include/sg.h) over sg_handle.
 #include "sg_handle.h"
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 using namespace sg;
 using namespace sg::detail;
+
+// ---------------------------------------------------------------- autotuner
+// Resident or streamed sweeps for (device, L, field, lattice count rounded up
+// to a power of two): both run the same PT intervals on a scratch handle of
+// that shape (random couplings and spins, 24 temperatures or fewer, 2
+// replicas, PT every 16, the batched form's captured intervals) and the
+// faster one is kept for the life of the process.
+namespace {
+struct TuneRes {
+  bool streamed;
+  double resident_ms, streamed_ms;
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, bool, size_t>, TuneRes> g_tune;
+
+TuneRes tune_measure(int device, int L, bool field, size_t nlat) {
+  sg_params p;
+  std::memset(&p, 0, sizeof(p));
+  const int T = (int)std::max<size_t>(1, std::min<size_t>(24, nlat / 2));
+  const int G = (int)std::max<size_t>(1, nlat / (2 * (size_t)T));
+  p.L = L; p.n_samples = 64 * G; p.n_temps = T; p.n_replicas = 2;
+  p.T_min = 0.5; p.T_max = 2.0; p.H = field ? 0.5 : 0.0;
+  p.coupling_seed = 0x70E0001; p.spin_seed = 0x70E0002;
+  p.pt_every = 16; p.meas_every = 0; p.device = device; p.form = SG_FORM_BATCHED;
+  p.kernel = SG_KERNEL_RESIDENT;
+  sg_handle h;
+  h.create(p);
+  h.init_lattice();
+  double ms[2];
+  for (int mode = 0; mode < 2; ++mode) {
+    h.set_streamed(mode == 1);
+    h.run(32, nullptr, nullptr);  // warm-up: code, caches, captured graphs
+    const double before = h.kernel_ms;
+    h.run(64, nullptr, nullptr);
+    ms[mode] = h.kernel_ms - before;
+  }
+  return TuneRes{ms[1] < ms[0], ms[0], ms[1]};
+}
+}  // namespace
+
+namespace sg {
+namespace detail {
+bool tune_streamed(int device, int L, bool field, size_t n_lattices) {
+  if (!phase_kernel_supported(L)) return false;
+  size_t bucket = 1;
+  while (bucket < n_lattices) bucket <<= 1;
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  const auto key = std::make_tuple(device, L, field, bucket);
+  auto it = g_tune.find(key);
+  if (it == g_tune.end()) it = g_tune.emplace(key, tune_measure(device, L, field, bucket)).first;
+  return it->second.streamed;
+}
+}  // namespace detail
+}  // namespace sg
 
 // ---------------------------------------------------------------- C-ABI
 template <class F>
@@ -176,6 +234,32 @@ sg_status sg_get_timing(sg_handle *h, double *kernel_ms, double *flips) {
 
 sg_status sg_set_profiling(sg_handle *h, int32_t on) {
   return guarded(h, [&] { h->profiling = on != 0; });
+}
+
+sg_status sg_tune_kernel(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
+                         int32_t *kernel, double *resident_ms, double *streamed_ms) {
+  g_create_error.clear();
+  if (!kernel || n_lattices < 1 || L < 2 || L > kMaxL || (L & 1)) return SG_ERR_INVALID;
+  try {
+    if (!phase_kernel_supported(L)) {
+      *kernel = SG_KERNEL_RESIDENT;
+      if (resident_ms) *resident_ms = 0;
+      if (streamed_ms) *streamed_ms = 0;
+      return SG_OK;
+    }
+    tune_streamed(device, L, field != 0, (size_t)n_lattices);
+    size_t bucket = 1;
+    while (bucket < (size_t)n_lattices) bucket <<= 1;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    const TuneRes &r = g_tune.at(std::make_tuple(device, L, field != 0, bucket));
+    *kernel = r.streamed ? SG_KERNEL_STREAMED : SG_KERNEL_RESIDENT;
+    if (resident_ms) *resident_ms = r.resident_ms;
+    if (streamed_ms) *streamed_ms = r.streamed_ms;
+    return SG_OK;
+  } catch (const Fail &e) {
+    g_create_error = e.msg;
+    return e.code;
+  }
 }
 
 sg_status sg_get_kernel_stats(sg_handle *h, int32_t kind, int64_t *launches, double *total_ms,

@@ -152,4 +152,17 @@ def test_job_footprint_counts_every_block():
     N, G, R, T = 16 ** 3, 2, 2, 24
     spins = 8 * G * R * T * N
     J = 8 * G * 3 * N
-    assert spins + J < j.footprint() < 1.2 * (spins + J)
+    site_records = 8 * G * 6 * N          # couplings per site, checkerboard order
+    stash = 8 * G * R * T * (N // 4)      # streamed sweeps: Philox words of each quad
+    major = spins + J + site_records + stash
+    assert major < j.footprint() < 1.1 * major
+
+
+@pytest.mark.gpu
+def test_autotuner_measures_both_kernels_and_caches():
+    k, r, s = sg.tune_kernel(8, 64)
+    assert k == sg.SG_KERNEL_RESIDENT and r == 0 and s == 0     # no streamed kernel below L=12
+    k, r, s = sg.tune_kernel(32, 64)
+    assert k in (sg.SG_KERNEL_RESIDENT, sg.SG_KERNEL_STREAMED) and r > 0 and s > 0
+    assert (k == sg.SG_KERNEL_STREAMED) == (s < r)
+    assert sg.tune_kernel(32, 60) == (k, r, s)                  # same power-of-two bucket
