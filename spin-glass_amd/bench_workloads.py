@@ -84,7 +84,7 @@ def mixed(args):
            "batched_over_plain": None if plain is None else round(value / plain, 3),
            "parallelism": "1 GPU"}
     return _base("spin flips/ns, mixed-L batch", value, "spin flips/ns", dt, cfg,
-                 _valu_roofline(value, "k_block<8|12|16>, k_big<20|24>"),
+                 _valu_roofline(value, "k_block<8|12|16>, k_mid<20>, k_one<24> (per tuner)"),
                  steps=sweeps // PT_EVERY)
 
 
@@ -174,7 +174,7 @@ def serve(args):
            "form": "native serve loop: admission by footprint, continuous batching per L class",
            "parallelism": "1 GPU"}
     return _base("spin flips/ns and p50/p99 job latency, serving loop", value, "spin flips/ns",
-                 wall, cfg, _valu_roofline(value, "k_block<8|12|16>, k_big<20|24>"),
+                 wall, cfg, _valu_roofline(value, "k_block<8|12|16>, k_mid<20>, k_one<24> (per tuner)"),
                  steps=args.jobs)
 
 

@@ -341,6 +341,10 @@ struct sg_batch {
       for (int k = 0; k < 2; ++k) {
         gather(c, c.tasks[k], true, pick_tasks, k);
         upload(c, c.tasks[k]);
+        if (single_kernel_supported(c.L)) {   // PT in its own launch
+          gather(c, c.ptc[k], false, pick_ptc, k);
+          upload(c, c.ptc[k]);
+        }
       }
     }
     HIPCHK(hipEventRecord(c.up_ev, c.st));
@@ -397,6 +401,10 @@ struct sg_batch {
               HIPCHK(launch_phase(c.L, c.field, c.jt.d, c.chunks.d, c.chunks.n, col, it, mode,
                                   c.st));
             }
+        } else if (single_kernel_supported(c.L)) {
+          HIPCHK(launch_pt_chunks(c.jt.d, c.ptc[k].d, c.ptc[k].n, c.st));
+          HIPCHK(launch_single(c.L, c.field, c.jt.d, c.tasks[k].d, c.tasks[k].n, pe, emode,
+                               c.st));
         } else {
           HIPCHK(launch_block(c.L, c.field, c.jt.d, c.tasks[k].d, c.tasks[k].n, 1, pe, emode,
                               c.st));

@@ -391,6 +391,9 @@ struct sg_handle {
     if (npairs)
       HIPCHK(hipMemcpy(d_pairs, pd.data(), sizeof(PairDesc) * npairs, hipMemcpyHostToDevice));
 
+    // resident tasks: PT pairs of the parity and the lattices left over, or
+    // (single-lattice kernel) every lattice on its own, PT in its own launch
+    const bool single = single_kernel_supported(L);
     for (int par = 0; par < 2; ++par) {
       std::vector<TaskDesc> tv;
       task_off[par].assign(G + 1, 0);
@@ -401,7 +404,7 @@ struct sg_handle {
             TaskDesc k;
             k.job = 0;
             k.lat[0] = (uint32_t)lat_index(g, rho, t);
-            if ((t & 1) == par && t + 1 < T) {
+            if (!single && (t & 1) == par && t + 1 < T) {
               k.lat[1] = (uint32_t)lat_index(g, rho, t + 1);
               k.pair = (uint32_t)(((size_t)g * R + rho) * (T - 1) + t);
               t += 2;
@@ -622,13 +625,22 @@ struct sg_handle {
       return;
     }
     const bool per_group = form == SG_FORM_PLAIN;
+    const bool single = single_kernel_supported(L);
     for (int g = 0; g < (per_group ? G : 1); ++g) {
       const int a = per_group ? task_off[par][g] : 0;
       const int b = per_group ? task_off[par][g + 1] : task_off[par][G];
       const double fl = (double)seg * N * 64.0 * R * T * (per_group ? 1 : G);
+      if (single && do_pt) {
+        const int pa = per_group ? ptc_off[par][g] : 0;
+        const int pb = per_group ? ptc_off[par][g + 1] : ptc_off[par][G];
+        HIPCHK(launch_pt_chunks(d_self, d_ptc[par] + pa, pb - pa, st));
+      }
       timed(0, fl, [&] {
-        HIPCHK(launch_block(L, field, d_self, d_tasks[par] + a, b - a, do_pt, seg, want_energy,
-                            st));
+        if (single)
+          HIPCHK(launch_single(L, field, d_self, d_tasks[par] + a, b - a, seg, want_energy, st));
+        else
+          HIPCHK(launch_block(L, field, d_self, d_tasks[par] + a, b - a, do_pt, seg,
+                              want_energy, st));
       });
     }
     if (seg > 0) HIPCHK(launch_advance(d_self, 1, seg, st));
@@ -639,7 +651,10 @@ struct sg_handle {
       launch_interval_streamed(0, 0, 0, mode);
       return;
     }
-    HIPCHK(launch_block(L, field, d_self, d_tasks[0], task_off[0][G], 0, 0, mode, st));
+    if (single_kernel_supported(L))
+      HIPCHK(launch_single(L, field, d_self, d_tasks[0], task_off[0][G], 0, mode, st));
+    else
+      HIPCHK(launch_block(L, field, d_self, d_tasks[0], task_off[0][G], 0, 0, mode, st));
   }
 
   // Streamed interval: the PT launch, then one launch per colour and sweep;

@@ -6,7 +6,9 @@
 namespace sg {
 
 // a sweep kernel exists for every even L in [2, 48]: k_block (lattice in
-// LDS, couplings in VGPRs) up to L = 16, k_big above
+// LDS, couplings in VGPRs) up to L = 16, k_mid (two lattices in LDS,
+// couplings from the site records) at L = 18, 20, k_big (in place in global
+// memory) above
 bool sweep_kernel_supported(int L);
 
 // One work-group per task (one lattice, or the lattices of slots t and t+1
@@ -17,6 +19,14 @@ bool sweep_kernel_supported(int L);
 // slots, 2 writes U and K.
 hipError_t launch_block(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks,
                         int ntasks, int do_pt, int nsweeps, int want_energy, hipStream_t st);
+
+// ---- single-lattice resident sweeps (k_one): L = 22, 24, 26, where one
+// lattice fits in LDS and two do not.  Tasks: lat[0] = lattice descriptor,
+// lat[1] unused, job; no PT (the caller runs launch_pt_chunks first); then
+// nsweeps sweeps and, with want_energy, the energies as in launch_block.
+bool single_kernel_supported(int L);
+hipError_t launch_single(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
+                         int nsweeps, int want_energy, hipStream_t st);
 
 // Measures every (group, temperature) of md: folds the measurement into the
 // ObsAcc sums and, with write_records, writes the int32 records to out.

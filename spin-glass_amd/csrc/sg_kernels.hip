@@ -70,6 +70,14 @@ __device__ __forceinline__ u32x4 philox_dev(uint32_t c0, uint32_t c1, uint32_t c
   return r;
 }
 
+// Word i & 3 of a Philox block by selects: an indexed read of the block would
+// place it in memory (LDS or scratch).
+__device__ __forceinline__ uint32_t pick_word(const u32x4 &r, int i) {
+  const uint32_t lo = (i & 1) ? r.v[1] : r.v[0];
+  const uint32_t hi = (i & 1) ? r.v[3] : r.v[2];
+  return (i & 2) ? hi : lo;
+}
+
 // Address spaces of what a kernel reaches through its job table.  The table
 // holds generic pointers; accessed as such, every load and store would be a
 // flat access and no uniform load could become a scalar load.  Descriptors,
@@ -271,11 +279,17 @@ __device__ __forceinline__ void dpp_add32(uint32_t (&A)[NA]) {
 //     leaves the half totals in lanes 31 and 63;
 //  4. each lane reads its half's planes (readlane) and extracts its own bit:
 //     one integer per lane per tree wave, which wave 0 adds.
-// fold must hold NL*P0*NT words and red TW*NL*64 int32, both LDS.  Every
-// thread calls it; out is valid in wave 0 (lane = sample).
-template <int P0, int NT, int NL>
+// fold holds the first PS of the NL*P0 planes (NT words each) and fold2 the
+// rest; red holds TW*NL*64 int32; all LDS.  Every thread calls it; out is
+// valid in wave 0 (lane = sample).
+template <int NT, int PS>
+__device__ __forceinline__ uint64_t *fold_plane(uint64_t *fold, uint64_t *fold2, int idx) {
+  return idx < PS ? fold + idx * NT : fold2 + (idx - PS) * NT;
+}
+
+template <int P0, int NT, int NL, int PS = NL * P0>
 __device__ void wg_count_lanes(const uint64_t (&A)[NL][P0], uint64_t *fold, int32_t *red,
-                               int32_t (&out)[NL]) {
+                               int32_t (&out)[NL], uint64_t *fold2 = nullptr) {
   constexpr int NW = NT / 64;
   constexpr int TW = NW < 4 ? NW : 4;
   constexpr int FT = TW * 64;
@@ -287,7 +301,7 @@ __device__ void wg_count_lanes(const uint64_t (&A)[NL][P0], uint64_t *fold, int3
 #pragma unroll
   for (int a = 0; a < NL; ++a)
 #pragma unroll
-    for (int p = 0; p < P0; ++p) fold[(a * P0 + p) * NT + tid] = A[a][p];
+    for (int p = 0; p < P0; ++p) fold_plane<NT, PS>(fold, fold2, a * P0 + p)[tid] = A[a][p];
   __syncthreads();
   if (wave < TW) {
     uint32_t H[NL][PW];
@@ -295,14 +309,14 @@ __device__ void wg_count_lanes(const uint64_t (&A)[NL][P0], uint64_t *fold, int3
     for (int a = 0; a < NL; ++a) {
       uint64_t S[PF];
 #pragma unroll
-      for (int p = 0; p < PF; ++p) S[p] = p < P0 ? fold[(a * P0 + p) * NT + tid] : 0;
+      for (int p = 0; p < PF; ++p) S[p] = p < P0 ? A[a][p] : 0;
 #pragma unroll
       for (int k = 1; k < K; ++k) {
         const int i = tid + k * FT;
         if (i < NT) {
           uint64_t B[P0];
 #pragma unroll
-          for (int p = 0; p < P0; ++p) B[p] = fold[(a * P0 + p) * NT + i];
+          for (int p = 0; p < P0; ++p) B[p] = fold_plane<NT, PS>(fold, fold2, a * P0 + p)[i];
           vc_add<PF>(S, B);
         }
       }
@@ -540,7 +554,7 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   if (pt_here) {
     const uint32_t sw3 = (kTagPT << 28) | (pd.rt & 0x0FFFFFFFu);
     const u32x4 r = philox_dev((uint32_t)(lane >> 2), pt_b, pd.g, sw3, pd.key0, pd.key1);
-    const bool acc = pt_accept(pd.beta_a, pd.beta_b, ua, ub, ka, kb, pd.H, r.v[lane & 3]);
+    const bool acc = pt_accept(pd.beta_a, pd.beta_b, ua, ub, ka, kb, pd.H, pick_word(r, lane));
     m = __ballot(acc);
   }
   // accepted lanes trade their bits on the way into LDS
@@ -670,12 +684,10 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
 }
 
 // ------------------------------------------------------- large-lattice kernel
-// L >= 18: a lattice no longer fits the VGPR budget of one Philox quad per
-// thread, so each thread walks several quads (j = tid, tid + NT, ...), reads
-// the couplings of each site from L2, and draws the quad's Philox block in
-// each phase.  With IN_LDS the task's lattices are copied into LDS for the
-// launch (L = 18, 20: two lattices fit in 160 KB); otherwise they are updated
-// in place in global memory (L2/MALL-resident), which is coherent within the
+// L >= 22: two lattices no longer fit in LDS, so each thread walks several
+// quads (j = tid, tid + NT, ...), reads the couplings of each site from L2,
+// draws the quad's Philox block in each phase, and updates the lattices in
+// place in global memory (L2/MALL-resident), which is coherent within the
 // work-group because all its waves share one CU.  Same task / PT / energy
 // semantics as k_block.
 template <int L>
@@ -718,13 +730,11 @@ __device__ __forceinline__ void site_geometry(int h, int c, int (&nb)[6], int &s
   jn[5] = x + L * (y + L * zm) + 2 * G::N;
 }
 
-template <int L, bool FIELD, bool IN_LDS>
+template <int L, bool FIELD>
 __global__ __launch_bounds__(1024) void k_big(
     const JobPtrs *__restrict__ jt, const TaskDesc *__restrict__ tasks, int do_pt,
     int nsweeps, int want_energy) {
   using G = BigGeo<L>;
-  constexpr int LDS_WORDS = IN_LDS ? 2 * G::N : 1;
-  __shared__ uint64_t slat[LDS_WORDS];
   __shared__ uint64_t red[G::NW * 24];
   __shared__ unsigned long long smask;
 
@@ -738,18 +748,7 @@ __global__ __launch_bounds__(1024) void k_big(
   const LatDesc d0 = ld_uniform(jp.descs + tk.lat[0]);
   const LatDesc d1 = ld_uniform(jp.descs + (two ? tk.lat[1] : tk.lat[0]));
   const int tid = threadIdx.x;
-  const int nl = two ? 2 : 1;
-  uint64_t *gl[2] = {(uint64_t *)(spins + d0.spin_off), (uint64_t *)(spins + d1.spin_off)};
-  uint64_t *lat[2];
-  if (IN_LDS) {
-    lat[0] = slat;
-    lat[1] = slat + G::N;
-    for (int a = 0; a < nl; ++a)
-      for (int i = tid; i < G::N; i += G::NT) lat[a][i] = gl[a][i];
-  } else {
-    lat[0] = gl[0];
-    lat[1] = gl[1];
-  }
+  uint64_t *lat[2] = {(uint64_t *)(spins + d0.spin_off), (uint64_t *)(spins + d1.spin_off)};
   const SG_GLOBAL uint64_t *J = Jg + d0.j_off;
   ThrRec tr0{}, tr1{};
   uint32_t n0 = 0;
@@ -768,7 +767,7 @@ __global__ __launch_bounds__(1024) void k_big(
       const int l = tid;
       const uint32_t sw3 = (kTagPT << 28) | (p.rt & 0x0FFFFFFFu);
       const u32x4 r = philox_dev((uint32_t)(l >> 2), b, p.g, sw3, p.key0, p.key1);
-      const uint32_t rr = r.v[l & 3];
+      const uint32_t rr = pick_word(r, l);
       const SG_GLOBAL int32_t *Ea = energy + (size_t)p.e_a * 128, *Eb = energy + (size_t)p.e_b * 128;
       const bool acc = pt_accept(p.beta_a, p.beta_b, Ea[l], Eb[l], Ea[64 + l], Eb[64 + l], p.H, rr);
       const unsigned long long m = __ballot(acc);
@@ -877,11 +876,539 @@ __global__ __launch_bounds__(1024) void k_big(
       }
     }
   }
+}
 
-  if (IN_LDS) {
-    for (int a = 0; a < nl; ++a)
-      for (int i = tid; i < G::N; i += G::NT) gl[a][i] = lat[a][i];
+// ------------------------------------------------- mid-size resident kernel
+// L = 18, 20: the two lattices of a pair task still fit in LDS (4 NH words,
+// 125 KB at L = 20) but their couplings no longer fit in VGPRs beside them.
+// They are read from the coupling site records (6 words per site, three
+// 16-byte loads, L2-resident), one site ahead: the record of the next site
+// (across phases too) is requested before the current site is updated in
+// both lattices, and the barrier between phases waits for LDS traffic only,
+// so the load stays in flight across it.  A thread owns the Philox quads
+// j = tid and j = tid + NT (two quads per thread, NT = 768 at L = 18 and
+// 1024 at L = 20); each quad's block is drawn once per sweep and lattice,
+// words 0, 1 for colour 0 and 2, 3 for colour 1, as in k_block.  The quad's
+// sites h = j and h = j + N/4 share a column (x, y); their neighbours are
+// kept as offsets from h.  Same task, PT and energy semantics as k_block.
+template <int L>
+struct MidGeo {
+  static constexpr int N = L * L * L;
+  static constexpr int NH = N / 2;
+  static constexpr int Q = N / 4;
+  static constexpr int LH = L / 2;
+  static constexpr int PL = LH * L;                    // words per z-plane, one colour
+  static constexpr int NT = ((Q / 2 + 63) / 64) * 64;  // two quads per thread
+  static constexpr int NLOAD = (NH + NT - 1) / NT;
+  static constexpr int SP = LH & 1;                    // site 1 parity flip
+};
+
+// Work-group barrier that waits for this wave's LDS operations only: global
+// loads in flight stay in flight across it.  The empty asm statements keep
+// the compiler from moving memory operations across the barrier.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt, expcnt not waited for
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
+// The six bond words of record idx of the record array at Jsb (a group's
+// block, or its colour-c half): a uniform base and a 32-bit byte offset, so
+// the loads take the SGPR-base form and no 64-bit address per site is kept.
+__device__ __forceinline__ void site_record(const SG_GLOBAL char *__restrict__ Jsb, int idx,
+                                            bool valid, uint64_t (&jw)[6]) {
+  if (!valid) return;
+  uint32_t off = (uint32_t)idx * 48u;
+  asm volatile("" : "+v"(off));   // keeps base + zext(off)
+  const SG_GLOBAL u64x2 *r = (const SG_GLOBAL u64x2 *)(Jsb + off);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const u64x2 v = r[k];
+    jw[2 * k] = v.x;
+    jw[2 * k + 1] = v.y;
   }
+}
+
+template <int L, bool FIELD>
+__global__ __launch_bounds__(MidGeo<L>::NT) void k_mid(
+    const JobPtrs *__restrict__ jt, const TaskDesc *__restrict__ tasks, int do_pt,
+    int nsweeps, int want_energy) {
+  using G = MidGeo<L>;
+  static_assert(G::NT <= G::Q && G::Q <= 2 * G::NT, "two quads per thread");
+  __shared__ uint64_t latB[2][G::NH], latW[2][G::NH];
+  __shared__ int32_t redi[4 * 64];
+
+  const TaskDesc tk = ld_uniform(tasks + blockIdx.x);
+  const JobPtrs jp = ld_uniform(jt + tk.job);
+  SG_GLOBAL uint64_t *__restrict__ spins = to_global(jp.spins);
+  SG_GLOBAL int32_t *__restrict__ energy = to_global(jp.energy);
+  unsigned long long *__restrict__ counters = jp.counters;
+  const bool two = tk.lat[1] != kNone;
+  const LatDesc d0 = ld_uniform(jp.descs + tk.lat[0]);
+  const LatDesc d1 = ld_uniform(jp.descs + (two ? tk.lat[1] : tk.lat[0]));
+  const SG_GLOBAL char *__restrict__ Js = (const SG_GLOBAL char *)(to_global(jp.Js) + 2 * d0.j_off);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const bool act1 = tid + G::NT < G::Q;   // second quad present
+
+  const bool pt_here = do_pt && two && tk.pair != kNone;
+  int32_t ua = 0, ub = 0, ka = 0, kb = 0;
+  PairDesc pd{};
+  uint32_t pt_b = 0;
+  if (pt_here) {
+    pd = ld_uniform(jp.pairs + tk.pair);
+    const JobState js = ld_uniform(jp.js);
+    pt_b = js.sweep / js.pt_every;
+    const SG_GLOBAL int32_t *Ea = energy + (size_t)pd.e_a * 128, *Eb = energy + (size_t)pd.e_b * 128;
+    ua = Ea[lane]; ub = Eb[lane]; ka = Ea[64 + lane]; kb = Eb[64 + lane];
+  }
+
+  const SG_GLOBAL uint64_t *g0 = spins + d0.spin_off, *g1 = spins + d1.spin_off;
+  uint64_t w[G::NLOAD][4];
+#pragma unroll
+  for (int k = 0; k < G::NLOAD; ++k) {
+    const int i = tid + k * G::NT;
+    if (i < G::NH) {
+      w[k][0] = g0[i];
+      w[k][1] = g0[G::NH + i];
+      if (two) {
+        w[k][2] = g1[i];
+        w[k][3] = g1[G::NH + i];
+      }
+    }
+  }
+
+  // site (q, s): h = tid + q NT + s Q; the record of the first site
+  uint64_t jc[6], jn[6];
+  site_record(Js, tid, true, jc);
+
+  // neighbour offsets from h in the opposite colour's half.  x: one of the
+  // two x-neighbours shares the index h, the other is h +- 1 (wrapping within
+  // the row), depending on the site's parity (y + z + c) & 1; xo[q][c] holds
+  // the offsets (+x, -x) of site 0 at colour c, and site 1 at colour c uses
+  // those of colour c ^ SP.  y: dy[q] (+y, -y).  z: site 0 (z < L/2) has +z
+  // at +PL and -z at dz[q][0]; site 1 (z >= L/2) +z at dz[q][1], -z at -PL.
+  int xo[2][2][2], dy[2][2], dz[2][2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int j = tid + q * G::NT;
+    const int xh = j % G::LH, y = (j / G::LH) % L, z0 = j / G::PL;
+    const int dxp = xh == G::LH - 1 ? -(G::LH - 1) : 1;
+    const int dxm = xh == 0 ? G::LH - 1 : -1;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const bool odd = ((y + z0 + c) & 1) != 0;
+      xo[q][c][0] = odd ? dxp : 0;
+      xo[q][c][1] = odd ? 0 : dxm;
+    }
+    dy[q][0] = y == L - 1 ? -(L - 1) * G::LH : G::LH;
+    dy[q][1] = y == 0 ? (L - 1) * G::LH : -G::LH;
+    dz[q][0] = z0 == 0 ? (L - 1) * G::PL : -G::PL;
+    dz[q][1] = z0 + G::LH == L - 1 ? -(L - 1) * G::PL : G::PL;
+  }
+
+  uint64_t m = 0;
+  if (pt_here) {
+    const uint32_t sw3 = (kTagPT << 28) | (pd.rt & 0x0FFFFFFFu);
+    const u32x4 r = philox_dev((uint32_t)(lane >> 2), pt_b, pd.g, sw3, pd.key0, pd.key1);
+    const bool acc = pt_accept(pd.beta_a, pd.beta_b, ua, ub, ka, kb, pd.H, pick_word(r, lane));
+    m = __ballot(acc);
+  }
+#pragma unroll
+  for (int k = 0; k < G::NLOAD; ++k) {
+    const int i = tid + k * G::NT;
+    if (i < G::NH) {
+      if (two) {
+        const uint64_t db = (w[k][0] ^ w[k][2]) & m, dw = (w[k][1] ^ w[k][3]) & m;
+        latB[0][i] = w[k][0] ^ db;
+        latW[0][i] = w[k][1] ^ dw;
+        latB[1][i] = w[k][2] ^ db;
+        latW[1][i] = w[k][3] ^ dw;
+      } else {
+        latB[0][i] = w[k][0];
+        latW[0][i] = w[k][1];
+      }
+    }
+  }
+  ThrRec tr0{}, tr1{};
+  uint32_t n0 = 0;
+  if (nsweeps > 0) {
+    tr0 = ld_uniform(jp.thr + d0.thr_idx);
+    tr1 = ld_uniform(jp.thr + d1.thr_idx);
+    n0 = ld_uniform(jp.js).sweep;
+  }
+  lds_barrier();
+
+  // the six neighbours of site (q, s) at colour c, from the half opp
+#define MID_NEIGHBOURS(opp, q, s, c, h, nv)                                               \
+  do {                                                                                   \
+    const int cs_ = (c) ^ ((s) & G::SP);                                                 \
+    nv[0] = opp[h + xo[q][cs_][0]];                                                      \
+    nv[1] = opp[h + xo[q][cs_][1]];                                                      \
+    nv[2] = opp[h + dy[q][0]];                                                           \
+    nv[3] = opp[h + dy[q][1]];                                                           \
+    nv[4] = opp[h + ((s) == 0 ? G::PL : dz[q][1])];                                      \
+    nv[5] = opp[h + ((s) == 0 ? dz[q][0] : -G::PL)];                                     \
+  } while (0)
+
+  const uint32_t sw0 = (kTagMC << 28) | (d0.rt & 0x0FFFFFFFu);
+  const uint32_t sw1 = (kTagMC << 28) | (d1.rt & 0x0FFFFFFFu);
+  for (int it = 0; it < nsweeps; ++it) {
+    const uint32_t n = n0 + (uint32_t)it;
+    uint32_t r1[2] = {0, 0}, r23[2][2][2];   // [q][lattice][word 2, 3]
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          // the next site's record: site 1 of this quad, site 0 of quad 1, or
+          // the next phase's first site
+          if (s == 0) {
+            site_record(Js + c * G::NH * 48, tid + q * G::NT + G::Q, q == 0 || act1, jn);
+          } else if (q == 0) {
+            const int nidx = act1 ? c * G::NH + tid + G::NT : (1 - c) * G::NH + tid;
+            site_record(Js, nidx, true, jn);
+          } else {
+            site_record(Js + (1 - c) * G::NH * 48, tid, true, jn);
+          }
+          if (q == 0 || act1) {
+            const int h = tid + q * G::NT + s * G::Q;
+            uint32_t rw[2];
+            if (c == 0 && s == 0) {
+              const uint32_t j = (uint32_t)(tid + q * G::NT);
+              const u32x4 ra = philox_dev(j, n, d0.g, sw0, d0.key0, d0.key1);
+              u32x4 rb = ra;
+              if (two) rb = philox_dev(j, n, d1.g, sw1, d1.key0, d1.key1);
+              rw[0] = ra.v[0]; r1[0] = ra.v[1]; r23[q][0][0] = ra.v[2]; r23[q][0][1] = ra.v[3];
+              rw[1] = rb.v[0]; r1[1] = rb.v[1]; r23[q][1][0] = rb.v[2]; r23[q][1][1] = rb.v[3];
+            } else if (c == 0) {
+              rw[0] = r1[0];
+              rw[1] = r1[1];
+            } else {
+              rw[0] = r23[q][0][s];
+              rw[1] = r23[q][1][s];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+              if (a == 1 && !two) break;
+              uint64_t *mine = c == 0 ? latB[a] : latW[a];
+              const uint64_t *opp = c == 0 ? latW[a] : latB[a];
+              uint64_t nv[6];
+              MID_NEIGHBOURS(opp, q, s, c, h, nv);
+              uint64_t sv = mine[h];
+              update_site<L, FIELD>(sv, nv, jc, rw[a], a == 0 ? tr0 : tr1);
+              mine[h] = sv;
+            }
+          }
+#pragma unroll
+          for (int e = 0; e < 6; ++e) jc[e] = jn[e];
+        }
+      }
+      lds_barrier();
+    }
+  }
+
+  // energies of the final states (every bond has one black end); jc holds
+  // the record of colour 0, site (0, 0)
+  uint64_t AU[2][1][5] = {}, AK[2][1][4] = {};
+  if (want_energy) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        if (s == 0)
+          site_record(Js, tid + q * G::NT + G::Q, q == 0 || act1, jn);
+        else if (q == 0)
+          site_record(Js, tid + G::NT, act1, jn);
+        if (q == 0 || act1) {
+          const int h = tid + q * G::NT + s * G::Q;
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            if (a == 1 && !two) break;
+            uint64_t nv[6];
+            MID_NEIGHBOURS(latW[a], q, s, 0, h, nv);
+            const uint64_t sv = latB[a][h];
+            uint64_t u0, u1, u2;
+            bond_count(sv, nv, jc, u0, u1, u2);
+            vc_add3(AU[a][0], u0, u1, u2);
+            if (want_energy > 1) {
+              vc_add1(AK[a][0], sv);
+              vc_add1(AK[a][0], latW[a][h]);
+            }
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < 6; ++e) jc[e] = jn[e];
+      }
+    }
+  }
+#undef MID_NEIGHBOURS
+
+  {
+    SG_GLOBAL uint64_t *o0 = spins + d0.spin_off, *o1 = spins + d1.spin_off;
+    for (int i = tid; i < G::NH; i += G::NT) {
+      o0[i] = latB[0][i];
+      o0[G::NH + i] = latW[0][i];
+    }
+    if (two)
+      for (int i = tid; i < G::NH; i += G::NT) {
+        o1[i] = latB[1][i];
+        o1[G::NH + i] = latW[1][i];
+      }
+  }
+
+  if (want_energy) {
+    // the lattices have left: their black halves serve as the fold scratch
+    // (5 NT <= 2 NH words), one lattice and one count at a time
+    uint64_t *fold = &latB[0][0];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      if (a == 1 && !two) break;
+      const LatDesc &d = a == 0 ? d0 : d1;
+      __syncthreads();
+      int32_t U[1];
+      wg_count_lanes<5, G::NT, 1>(AU[a], fold, redi, U);
+      if (tid < 64) energy[(size_t)d.e_idx * 128 + tid] = U[0];
+      if (want_energy > 1) {
+        __syncthreads();
+        int32_t K[1];
+        wg_count_lanes<4, G::NT, 1>(AK[a], fold, redi, K);
+        if (tid < 64) energy[(size_t)d.e_idx * 128 + 64 + tid] = K[0];
+      }
+    }
+  }
+  if (pt_here && tid == 0) atomicAdd(&counters[pd.cnt_idx], (unsigned long long)__popcll(m));
+}
+
+// ------------------------------------------- single-lattice resident kernel
+// L = 22, 24, 26: one lattice fits in LDS (N words, 137 KB at L = 26), two
+// do not, so a task is one lattice and its PT step runs before the launch
+// (k_pt_chunk over its pairs).  Thread t owns the column col = t mod PL
+// (x/2, y fixed) and the z-run of QZ quads z0 = zg QZ + k, zg = t / PL: its x
+// and y neighbour offsets are fixed per thread, the site parity alternates
+// with k, and only the quads at z0 = 0 and z0 = L/2 - 1 have z wraps.  Per
+// sweep each quad's Philox block is drawn once (words 2, 3 kept for colour
+// 1); the coupling records of the next quad (or of the next phase's first
+// quad) are requested before the current quad is updated, so one quad's
+// records are in flight while another's are used.  Energies as in k_block; the fold
+// scratch of the reduction spans both colour halves.
+template <int L>
+struct OneGeo {
+  static constexpr int N = L * L * L;
+  static constexpr int NH = N / 2;
+  static constexpr int Q = N / 4;
+  static constexpr int LH = L / 2;
+  static constexpr int PL = LH * L;                           // columns
+  // z-runs per column: as many as 1024 threads allow, except at L = 24,
+  // where three runs (864 threads, 14 waves) are held to 128 VGPRs and
+  // spill; two runs (576 threads, up to 168 VGPRs) measure faster there
+  static constexpr int ZG = L == 24 ? 2 : (1024 / PL < LH ? 1024 / PL : LH);
+  static constexpr int QZ = (LH + ZG - 1) / ZG;               // quads per thread
+  static constexpr int NA = PL * ZG;                          // threads with quads
+  static constexpr int NT = ((NA + 63) / 64) * 64;
+  static constexpr int NLOAD = (NH + NT - 1) / NT;
+  static constexpr int SP = LH & 1;                           // site 1 parity flip
+  static constexpr int PU = ceil_log2(12 * QZ + 1);           // per-thread U bits
+  static constexpr int PK = ceil_log2(4 * QZ + 1);            // per-thread K bits
+};
+
+template <int L, bool FIELD>
+__global__ __launch_bounds__(OneGeo<L>::NT) void k_one(const JobPtrs *__restrict__ jt,
+                                                       const TaskDesc *__restrict__ tasks,
+                                                       int nsweeps, int want_energy) {
+  using G = OneGeo<L>;
+  static_assert(((G::PU + 1) / 2) * G::NT <= G::NH && ((G::PK + 1) / 2) * G::NT <= G::NH,
+                "fold scratch within the colour halves");
+  __shared__ uint64_t latB[G::NH], latW[G::NH];
+  __shared__ int32_t redi[4 * 64];
+
+  const TaskDesc tk = ld_uniform(tasks + blockIdx.x);
+  const JobPtrs jp = ld_uniform(jt + tk.job);
+  SG_GLOBAL uint64_t *__restrict__ spins = to_global(jp.spins);
+  SG_GLOBAL int32_t *__restrict__ energy = to_global(jp.energy);
+  const LatDesc d = ld_uniform(jp.descs + tk.lat[0]);
+  const SG_GLOBAL char *__restrict__ Js = (const SG_GLOBAL char *)(to_global(jp.Js) + 2 * d.j_off);
+  const int tid = threadIdx.x;
+  const bool act = tid < G::NA;
+  const int col = act ? tid % G::PL : 0, zg = act ? tid / G::PL : 0;
+
+  SG_GLOBAL uint64_t *g = spins + d.spin_off;
+  uint64_t w[G::NLOAD][2];
+#pragma unroll
+  for (int k = 0; k < G::NLOAD; ++k) {
+    const int i = tid + k * G::NT;
+    if (i < G::NH) {
+      w[k][0] = g[i];
+      w[k][1] = g[G::NH + i];
+    }
+  }
+
+  // quad k of the thread: j = col + PL z0, valid while z0 < L/2 (k < nq);
+  // its sites h = j + s Q, s = 0, 1
+  const int jb = col + G::PL * zg * G::QZ;
+  const int nq = act ? G::LH - zg * G::QZ : 0;
+#define quad_ok(k) ((k) < nq)
+  // records of both sites of quad k at colour c
+#define ONE_RECORDS(c, k, jw)                                                             \
+  do {                                                                                   \
+    site_record(Js + (c) * G::NH * 48, jb + (k) * G::PL, quad_ok(k), jw[0]);             \
+    site_record(Js + (c) * G::NH * 48, jb + (k) * G::PL + G::Q, quad_ok(k), jw[1]);      \
+  } while (0)
+  uint64_t jc[2][6], jn[2][6];
+  ONE_RECORDS(0, 0, jc);
+
+  // x offsets by site parity (1: +x at h + dxp, -x at h; 0: +x at h, -x at
+  // h + dxm), XO[b] for parity p0 ^ b, p0 the parity of the column's first
+  // site at colour 0; y offsets; z offsets of the wrapping quads
+  const int xh = col % G::LH, y = col / G::LH;
+  const int dxp = xh == G::LH - 1 ? -(G::LH - 1) : 1;
+  const int dxm = xh == 0 ? G::LH - 1 : -1;
+  const bool p0 = ((y + zg * G::QZ) & 1) != 0;
+  int XO[2][2];
+  XO[0][0] = p0 ? dxp : 0;
+  XO[0][1] = p0 ? 0 : dxm;
+  XO[1][0] = p0 ? 0 : dxp;
+  XO[1][1] = p0 ? dxm : 0;
+  const int dyp = y == L - 1 ? -(L - 1) * G::LH : G::LH;
+  const int dym = y == 0 ? (L - 1) * G::LH : -G::LH;
+  const int dz_first = zg == 0 ? (L - 1) * G::PL : -G::PL;    // -z of site 0, quad 0
+
+#pragma unroll
+  for (int k = 0; k < G::NLOAD; ++k) {
+    const int i = tid + k * G::NT;
+    if (i < G::NH) {
+      latB[i] = w[k][0];
+      latW[i] = w[k][1];
+    }
+  }
+  ThrRec tr{};
+  uint32_t n0 = 0;
+  if (nsweeps > 0) {
+    tr = ld_uniform(jp.thr + d.thr_idx);
+    n0 = ld_uniform(jp.js).sweep;
+  }
+  lds_barrier();
+
+  // the six neighbours of site s of quad k at colour c, from the half opp
+#define ONE_NEIGHBOURS(opp, k, s, c, h, nv)                                               \
+  do {                                                                                   \
+    const int b_ = ((k) ^ (c) ^ ((s) & G::SP)) & 1;                                      \
+    nv[0] = opp[h + XO[b_][0]];                                                          \
+    nv[1] = opp[h + XO[b_][1]];                                                          \
+    nv[2] = opp[h + dyp];                                                                \
+    nv[3] = opp[h + dym];                                                                \
+    if ((s) == 0) {                                                                      \
+      nv[4] = opp[h + G::PL];                                                            \
+      nv[5] = opp[h + ((k) == 0 ? dz_first : -G::PL)];                                   \
+    } else {                                                                             \
+      nv[4] = opp[h + (zg * G::QZ + (k) == G::LH - 1 ? -(L - 1) * G::PL : G::PL)];       \
+      nv[5] = opp[h - G::PL];                                                            \
+    }                                                                                    \
+  } while (0)
+
+  const uint32_t sw = (kTagMC << 28) | (d.rt & 0x0FFFFFFFu);
+  for (int it = 0; it < nsweeps; ++it) {
+    const uint32_t n = n0 + (uint32_t)it;
+    uint32_t r23[G::QZ][2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      uint64_t *mine = c == 0 ? latB : latW;
+      const uint64_t *opp = c == 0 ? latW : latB;
+#pragma unroll
+      for (int k = 0; k < G::QZ; ++k) {
+        // the next quad's records (the next phase's first quad after the last)
+        if (k + 1 < G::QZ)
+          ONE_RECORDS(c, k + 1, jn);
+        else
+          ONE_RECORDS(1 - c, 0, jn);
+        if (quad_ok(k)) {
+          const int j = jb + k * G::PL;
+          uint32_t rw[2];
+          if (c == 0) {
+            const u32x4 r = philox_dev((uint32_t)j, n, d.g, sw, d.key0, d.key1);
+            rw[0] = r.v[0];
+            rw[1] = r.v[1];
+            r23[k][0] = r.v[2];
+            r23[k][1] = r.v[3];
+          } else {
+            rw[0] = r23[k][0];
+            rw[1] = r23[k][1];
+          }
+#pragma unroll
+          for (int s = 0; s < 2; ++s) {
+            const int h = j + s * G::Q;
+            uint64_t nv[6];
+            ONE_NEIGHBOURS(opp, k, s, c, h, nv);
+            uint64_t sv = mine[h];
+            update_site<L, FIELD>(sv, nv, jc[s], rw[s], tr);
+            mine[h] = sv;
+          }
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+          for (int e = 0; e < 6; ++e) jc[s][e] = jn[s][e];
+      }
+      lds_barrier();
+    }
+  }
+
+  // energies of the final state; jc holds the records of colour 0, quad 0
+  uint64_t AU[1][G::PU] = {}, AK[1][G::PK] = {};
+  if (want_energy) {
+#pragma unroll
+    for (int k = 0; k < G::QZ; ++k) {
+      if (k + 1 < G::QZ) ONE_RECORDS(0, k + 1, jn);
+      if (quad_ok(k)) {
+        const int j = jb + k * G::PL;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const int h = j + s * G::Q;
+          uint64_t nv[6];
+          ONE_NEIGHBOURS(latW, k, s, 0, h, nv);
+          const uint64_t sv = latB[h];
+          uint64_t u0, u1, u2;
+          bond_count(sv, nv, jc[s], u0, u1, u2);
+          vc_add3(AU[0], u0, u1, u2);
+          if (want_energy > 1) {
+            vc_add1(AK[0], sv);
+            vc_add1(AK[0], latW[h]);
+          }
+        }
+      }
+      if (k + 1 < G::QZ)
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+          for (int e = 0; e < 6; ++e) jc[s][e] = jn[s][e];
+    }
+  }
+#undef ONE_NEIGHBOURS
+#undef ONE_RECORDS
+
+  for (int i = tid; i < G::NH; i += G::NT) {
+    g[i] = latB[i];
+    g[G::NH + i] = latW[i];
+  }
+
+  if (want_energy) {
+    // the lattice has left: its halves serve as the fold scratch
+    constexpr int PSU = (G::PU + 1) / 2, PSK = (G::PK + 1) / 2;
+    __syncthreads();
+    int32_t U[1];
+    wg_count_lanes<G::PU, G::NT, 1, PSU>(AU, latB, redi, U, latW);
+    if (tid < 64) energy[(size_t)d.e_idx * 128 + tid] = U[0];
+    if (want_energy > 1) {
+      __syncthreads();
+      int32_t K[1];
+      wg_count_lanes<G::PK, G::NT, 1, PSK>(AK, latB, redi, K, latW);
+      if (tid < 64) energy[(size_t)d.e_idx * 128 + 64 + tid] = K[0];
+    }
+  }
+#undef quad_ok
 }
 
 // ---------------------------------------------------------- measure kernel
@@ -1216,7 +1743,7 @@ __global__ __launch_bounds__(256) void k_pt_chunk(const JobPtrs *__restrict__ jt
   const u32x4 r = philox_dev((uint32_t)(lane >> 2), b, p.g, (kTagPT << 28) | (p.rt & 0x0FFFFFFFu),
                              p.key0, p.key1);
   const bool acc = pt_accept(p.beta_a, p.beta_b, Ea[lane], Eb[lane], Ea[64 + lane], Eb[64 + lane],
-                             p.H, r.v[lane & 3]);
+                             p.H, pick_word(r, lane));
   const uint64_t m = __ballot(acc);
   if (m) {
 #pragma unroll
@@ -1261,18 +1788,49 @@ hipError_t launch_block(int L, bool field, const JobPtrs *jt, const TaskDesc *ta
     return hipGetLastError();
     SG_LDS_SIZES(X)
 #undef X
-#define X(l, in_lds)                                                                     \
+#define X(l)                                                                             \
   case l:                                                                                \
     if (field)                                                                           \
-      hipLaunchKernelGGL((k_big<l, true, in_lds>), dim3(ntasks), dim3(1024), 0, st,      \
+      hipLaunchKernelGGL((k_mid<l, true>), dim3(ntasks), dim3(MidGeo<l>::NT), 0, st,     \
                          jt, tasks, do_pt, nsweeps, want_energy);                        \
     else                                                                                 \
-      hipLaunchKernelGGL((k_big<l, false, in_lds>), dim3(ntasks), dim3(1024), 0, st,     \
+      hipLaunchKernelGGL((k_mid<l, false>), dim3(ntasks), dim3(MidGeo<l>::NT), 0, st,    \
                          jt, tasks, do_pt, nsweeps, want_energy);                        \
     return hipGetLastError();
-    X(18, true) X(20, true) X(22, false) X(24, false) X(26, false) X(28, false)
-    X(30, false) X(32, false) X(34, false) X(36, false) X(38, false) X(40, false)
-    X(42, false) X(44, false) X(46, false) X(48, false)
+    X(18) X(20)
+#undef X
+#define X(l)                                                                             \
+  case l:                                                                                \
+    if (field)                                                                           \
+      hipLaunchKernelGGL((k_big<l, true>), dim3(ntasks), dim3(1024), 0, st,              \
+                         jt, tasks, do_pt, nsweeps, want_energy);                        \
+    else                                                                                 \
+      hipLaunchKernelGGL((k_big<l, false>), dim3(ntasks), dim3(1024), 0, st,             \
+                         jt, tasks, do_pt, nsweeps, want_energy);                        \
+    return hipGetLastError();
+    X(22) X(24) X(26) X(28) X(30) X(32) X(34) X(36) X(38) X(40) X(42) X(44) X(46) X(48)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+bool single_kernel_supported(int L) { return L == 22 || L == 24 || L == 26; }
+
+hipError_t launch_single(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
+                         int nsweeps, int want_energy, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  switch (L) {
+#define X(l)                                                                             \
+  case l:                                                                                \
+    if (field)                                                                           \
+      hipLaunchKernelGGL((k_one<l, true>), dim3(ntasks), dim3(OneGeo<l>::NT), 0, st,     \
+                         jt, tasks, nsweeps, want_energy);                               \
+    else                                                                                 \
+      hipLaunchKernelGGL((k_one<l, false>), dim3(ntasks), dim3(OneGeo<l>::NT), 0, st,    \
+                         jt, tasks, nsweeps, want_energy);                               \
+    return hipGetLastError();
+    X(22) X(24) X(26)
 #undef X
     default:
       return hipErrorInvalidValue;

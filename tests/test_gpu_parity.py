@@ -64,9 +64,12 @@ def test_sweeps_pt_measurements_match_oracle(L):
     np.testing.assert_allclose(sim.results(), ref, rtol=1e-12, atol=0)
 
 
-@pytest.mark.parametrize("L", [18, 20, 22, 24, 32, 48])
+@pytest.mark.parametrize("L", [18, 20, 22, 24, 26, 28, 32, 48])
 def test_large_lattice_sweeps_match_oracle(L):
-    """L > 16 runs the large-lattice kernel: LDS-resident for L = 18, 20,
+    """L > 16 runs the resident large-lattice kernels: two lattices in LDS
+    with couplings from the site records for L = 18, 20 (site 1 of a quad has
+    the opposite parity at L = 18, the same at L = 20); one lattice in LDS
+    and PT in its own launch for L = 22, 24, 26 (uneven z-runs at 22 and 26);
     updated in place in global memory above; 80 samples = one full and one
     partial lane group; 4 temperatures give pair-only and mixed task lists."""
     betas = geometric_betas(0.3, 2.0, 4)
@@ -127,7 +130,7 @@ def test_resident_and_streamed_kernels_agree_in_both_forms():
         np.testing.assert_array_equal(r[3], runs[0][3])
 
 
-@pytest.mark.parametrize("L", [20, 26])
+@pytest.mark.parametrize("L", [18, 20, 22, 24, 26])
 def test_large_lattice_field_matches_oracle(L):
     betas = geometric_betas(0.3, 2.0, 3)
     sim, orc = _pair(L, 64, 2, betas, H=0.5, pt_every=1, meas_every=3,
