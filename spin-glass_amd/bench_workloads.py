@@ -120,7 +120,7 @@ def large(args):
     cfg = {"workload": f"L=32 and L=48 (separately), 64 samples, 32 temperatures geometric in "
                        f"T=[0.3,2.0], 2 replicas, PT every {PT_EVERY}, {sweeps} sweeps each",
            "per_L": per_l, "form": "batched", "parallelism": "1 GPU"}
-    r = _valu_roofline(value, "k_phase<32|48,false> (streamed)")
+    r = _valu_roofline(value, "k_slab<32,false>, k_phase<48,false> (streamed)")
     r["hbm"] = {L: v["hbm_frac"] for L, v in per_l.items()}
     return _base("spin flips/ns, L=32 and 48", value, "spin flips/ns", tot_s, cfg, r,
                  steps=2 * sweeps // PT_EVERY)

@@ -154,6 +154,7 @@ struct JobPtrs {
   JobState *js;
   int32_t *energy;
   unsigned long long *counters;
+  uint64_t *spins2;    // slab sweeps: the second spin buffer (same layout), or null
 };
 
 // One work-group task of the sweep kernel: lattice descriptor indices (the

@@ -80,6 +80,7 @@ struct Class {
   DevList<JobPtrs> jt;
   DevList<TaskDesc> tasks[2];   // resident: pair/single tasks
   DevList<TaskDesc> chunks;     // streamed: (lattice, chunk) tasks
+  DevList<TaskDesc> slabs;      // streamed with a slab kernel: (lattice, slab pair) tasks
   DevList<TaskDesc> ptc[2];     // streamed: (pair, chunk) tasks of the PT launch
   hipEvent_t up_ev = nullptr;   // the last upload from the staging
   bool uploaded = false;
@@ -87,6 +88,7 @@ struct Class {
   void free_all() {
     jt.free_all();
     chunks.free_all();
+    slabs.free_all();
     for (int k = 0; k < 2; ++k) {
       tasks[k].free_all();
       ptc[k].free_all();
@@ -312,6 +314,7 @@ struct sg_batch {
 
   static const std::vector<TaskDesc> &pick_tasks(sg_handle &h, int par) { return h.h_tasks[par]; }
   static const std::vector<TaskDesc> &pick_chunks(sg_handle &h, int) { return h.h_chunks; }
+  static const std::vector<TaskDesc> &pick_slabs(sg_handle &h, int) { return h.h_slabs; }
   static const std::vector<TaskDesc> &pick_ptc(sg_handle &h, int par) { return h.h_ptc[par]; }
 
   void build(Class &c) {
@@ -333,6 +336,10 @@ struct sg_batch {
     if (c.streamed) {
       gather(c, c.chunks, false, pick_chunks, 0);
       upload(c, c.chunks);
+      if (slab_kernel_supported(c.L)) {
+        gather(c, c.slabs, false, pick_slabs, 0);
+        upload(c, c.slabs);
+      }
       for (int k = 0; k < 2; ++k) {
         gather(c, c.ptc[k], false, pick_ptc, k);
         upload(c, c.ptc[k]);
@@ -393,10 +400,20 @@ struct sg_batch {
         if (c.streamed) {
           const int pm = kPhEnergy | (emode > 1 ? kPhEnergyK : 0);
           HIPCHK(launch_pt_chunks(c.jt.d, c.ptc[k].d, c.ptc[k].n, c.st));
-          for (int it = 0; it < pe; ++it)
+          // pairs of sweeps by the slab kernel, a last odd sweep by the
+          // colour launches
+          const int ns = slab_kernel_supported(c.L) ? pe & ~1 : 0;
+          for (int it = 0; it < ns; ++it) {
+            int mode = kPhUpdate;
+            if (it == 0 && ns == pe) mode |= kPhZero;
+            if (it == pe - 1) mode |= pm;
+            HIPCHK(launch_slab(c.L, c.field, c.jt.d, c.slabs.d, c.slabs.n, it & 1, it, mode,
+                               c.st));
+          }
+          for (int it = ns; it < pe; ++it)
             for (int col = 0; col < 2; ++col) {
               int mode = kPhUpdate;
-              if (it == 0 && col == 0) mode |= kPhZero;
+              if (it == ns && col == 0) mode |= kPhZero;
               if (it == pe - 1 && col == 1) mode |= pm;
               HIPCHK(launch_phase(c.L, c.field, c.jt.d, c.chunks.d, c.chunks.n, col, it, mode,
                                   c.st));

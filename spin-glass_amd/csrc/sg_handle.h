@@ -119,6 +119,9 @@ inline size_t job_footprint(int L, int n_samples, int R, int T) {
     // chunk tasks: XCD padding adds at most 7 lattices' worth per group
     b += r(8 * nlat * N / 4) + r(sizeof(TaskDesc) * (nlat + 7 * G) * phase_chunks(L)) +
          2 * r(sizeof(TaskDesc) * (npairs / 2 + G * R) * pt_chunks(L));
+  if (streamed && slab_kernel_supported(L))
+    // slab sweeps: the second spin buffer and the (lattice, slab pair) tasks
+    b += r(8 * nlat * N) + r(sizeof(TaskDesc) * (nlat + 7 * G) * kSlabParts);
   return b;
 }
 
@@ -186,6 +189,12 @@ struct sg_handle {
   TaskDesc *d_chunks = nullptr, *d_ptc[2] = {nullptr, nullptr};
   std::vector<TaskDesc> h_chunks, h_ptc[2];
   std::vector<int> chunk_off, ptc_off[2];
+  // slab sweeps (L with a slab kernel): second spin buffer, (lattice, slab
+  // pair) tasks per group like the chunks
+  uint64_t *d_spins2 = nullptr;
+  TaskDesc *d_slabs = nullptr;
+  std::vector<TaskDesc> h_slabs;
+  std::vector<int> slab_off;
 
   uint64_t *d_spins = nullptr, *d_J = nullptr;
   ThrRec *d_thr = nullptr;
@@ -276,6 +285,7 @@ struct sg_handle {
     dev_free(d_lat); dev_free(d_pairs); dev_free(d_meas); dev_free(d_jobs); dev_free(d_cnt);
     dev_free(d_acc); dev_free(d_cs); dev_free(d_tasks[0]); dev_free(d_tasks[1]);
     dev_free(d_self); dev_free(d_Js); dev_free(d_rng); dev_free(d_chunks); dev_free(d_ptc[0]); dev_free(d_ptc[1]);
+    dev_free(d_spins2); dev_free(d_slabs);
     for (auto &g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();
     for (int i = 0; i < kRing; ++i) {
@@ -459,8 +469,13 @@ struct sg_handle {
     if (can_stream) {
       build_stream_tasks();
       dev_alloc(&d_rng, n_lat() * (size_t)(N / 4));
+      if (slab_kernel_supported(L)) {
+        build_slab_tasks();
+        dev_alloc(&d_spins2, n_lat() * N);
+      }
     }
     self.spins = d_spins; self.J = d_J; self.Js = d_Js; self.rng = d_rng;
+    self.spins2 = d_spins2;
     self.descs = d_lat; self.pairs = d_pairs;
     self.thr = d_thr; self.js = d_jobs; self.energy = d_energy; self.counters = d_cnt;
     dev_alloc(&d_self, 1);
@@ -520,6 +535,29 @@ struct sg_handle {
         HIPCHK(hipMemcpy(d_ptc[par], h_ptc[par].data(), sizeof(TaskDesc) * h_ptc[par].size(),
                          hipMemcpyHostToDevice));
     }
+  }
+
+  void build_slab_tasks() {
+    h_slabs.clear();
+    slab_off.assign(G + 1, 0);
+    const bool per_group = form == SG_FORM_PLAIN;
+    std::vector<TaskDesc> run;
+    for (int g = 0; g < G; ++g) {
+      if (per_group) slab_off[g] = (int)h_slabs.size();
+      for (int rho = 0; rho < R; ++rho)
+        for (int t = 0; t < T; ++t)
+          for (int k = 0; k < kSlabParts; ++k)
+            run.push_back(TaskDesc{{(uint32_t)lat_index(g, rho, t), (uint32_t)k}, kNoPair, 0});
+      if (per_group || g == G - 1) {
+        const std::vector<TaskDesc> o = xcd_order(run, (uint32_t)kSlabParts);
+        h_slabs.insert(h_slabs.end(), o.begin(), o.end());
+        run.clear();
+      }
+    }
+    slab_off[G] = (int)h_slabs.size();
+    dev_alloc(&d_slabs, h_slabs.size());
+    HIPCHK(hipMemcpy(d_slabs, h_slabs.data(), sizeof(TaskDesc) * h_slabs.size(),
+                     hipMemcpyHostToDevice));
   }
 
   // coupling site records from the [3][N] couplings (after every change of J)
@@ -676,11 +714,22 @@ struct sg_handle {
         continue;
       }
       const double fl = (double)seg * N * 64.0 * R * T * (per_group ? 1 : G);
+      // pairs of sweeps by the slab kernel (buffer 1 and back), a last odd
+      // sweep by the colour launches
+      const int ns = d_slabs ? seg & ~1 : 0;
+      const int sa = per_group && d_slabs ? slab_off[g] : 0;
+      const int sb = d_slabs ? (per_group ? slab_off[g + 1] : slab_off[G]) : 0;
       timed(0, fl, [&] {
-        for (int it = 0; it < seg; ++it)
+        for (int it = 0; it < ns; ++it) {
+          int mode = kPhUpdate;
+          if (it == 0 && want_energy && ns == seg) mode |= kPhZero;
+          if (it == seg - 1) mode |= emode;
+          HIPCHK(launch_slab(L, field, d_self, d_slabs + sa, sb - sa, it & 1, it, mode, st));
+        }
+        for (int it = ns; it < seg; ++it)
           for (int c = 0; c < 2; ++c) {
             int mode = kPhUpdate;
-            if (it == 0 && c == 0 && want_energy) mode |= kPhZero;
+            if (it == ns && c == 0 && want_energy) mode |= kPhZero;
             if (it == seg - 1 && c == 1) mode |= emode;
             HIPCHK(launch_phase(L, field, d_self, d_chunks + ca, cb - ca, c, it, mode, st));
           }

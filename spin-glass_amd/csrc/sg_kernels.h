@@ -53,6 +53,16 @@ bool phase_kernel_supported(int L);
 // mode = kPh* bits (energy bits with c = 1 only).
 hipError_t launch_phase(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
                         int c, int it, int mode, hipStream_t st);
+// ---- slab sweeps of streamed lattices: one launch per sweep (k_slab)
+constexpr int kSlabParts = 4;     // slab pairs per lattice (tasks per lattice)
+bool slab_kernel_supported(int L);
+// Tasks: lat[0] = lattice descriptor, lat[1] = slab pair (>= kSlabParts:
+// padding, no work), job.  Reads spins (dir = 0) or spins2 (dir = 1) and
+// writes the other buffer; it = sweep within the interval; mode = kPh* bits
+// (kPhZero in an earlier launch than the energy bits).
+hipError_t launch_slab(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
+                       int dir, int it, int mode, hipStream_t st);
+
 // Tasks: lat[0] = PairDesc index, lat[1] = chunk, job.
 hipError_t launch_pt_chunks(const JobPtrs *jt, const TaskDesc *tasks, int ntasks, hipStream_t st);
 // coupling site records of one group from its [3][N] couplings

@@ -1717,6 +1717,229 @@ __global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ 
   }
 }
 
+// ------------------------------------------------------------- slab kernel
+// Streamed lattices, one launch per sweep.  A task is (lattice, slab pair p):
+// the owned planes are z in [za, za + S) and their images z + L/2 (za = pS,
+// S = L/8, four slab pairs per lattice), so that each owned Philox quad's
+// four words are used by the task that draws them.  Phase 1 updates colour
+// 0 on the owned planes and one halo plane on each side, into LDS; the halo
+// updates recompute, bit for bit, what the neighbouring task computes for
+// its own planes (same counter-based random numbers, same inputs).  Phase 2
+// updates colour 1 on the owned planes from the new colour 0 in LDS.  The
+// sweep reads one spin buffer and writes the other (dir selects which), so
+// no task reads a word that another task of the launch writes.  The
+// colour-1 planes the task reads (owned plus two halo planes each side) are
+// staged in LDS first; the kernel exists where both colours' planes fit
+// (L = 32: 112 KB).  Where they do not (L = 40, 48), reading colour 1 from
+// L2 in the item loop measured slower than the colour launches (k_phase),
+// which remain the streamed kernel there.  Energies as in k_phase: zeroed by
+// one launch (kPhZero, task p = 0), accumulated with per-lane atomics by a
+// later one (colour-1 sites: every bond once).
+template <int L>
+struct SlabGeo {
+  static constexpr int N = L * L * L;
+  static constexpr int NH = N / 2;
+  static constexpr int LH = L / 2;
+  static constexpr int PL = LH * L;                     // columns (words per colour plane)
+  static constexpr int NP = kSlabParts;                 // slab pairs per lattice
+  static constexpr int S = LH / NP;                     // owned planes per half
+  static constexpr int E = S + 2;                       // colour-0 planes per half, with halo
+  static constexpr int CPT = (PL + 1023) / 1024;        // columns per thread
+  static constexpr int NT = PL / CPT;                   // every thread has its columns
+  static constexpr int C1W = 2 * (S + 4) * PL;          // staged colour-1 words
+  static constexpr int PU = ceil_log2(12 * S * CPT + 1);
+  static constexpr int PK = ceil_log2(4 * S * CPT + 1);
+  static_assert(LH % NP == 0 && PL % CPT == 0, "slab shape");
+  static_assert((2 * E * PL + C1W) * 8 <= 150 * 1024, "both colours' planes in LDS");
+};
+
+// A 32-bit byte offset the compiler cannot see through (a pure asm, free to
+// move): loads from a uniform base at such an offset take the SGPR-base form
+// and no 64-bit address per item is formed or kept.
+__device__ __forceinline__ uint32_t opaque(uint32_t v) {
+  asm("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ uint64_t gload(const SG_GLOBAL uint64_t *base, int idx) {
+  return *(const SG_GLOBAL uint64_t *)((const SG_GLOBAL char *)base + opaque((uint32_t)idx * 8u));
+}
+
+// site record idx of Jsb without the offset barrier: the compiler may issue
+// these loads as early as it likes (k_slab's items are independent)
+__device__ __forceinline__ void site_record_free(const SG_GLOBAL char *__restrict__ Jsb, int idx,
+                                                 uint64_t (&jw)[6]) {
+  const SG_GLOBAL u64x2 *r = (const SG_GLOBAL u64x2 *)(Jsb + opaque((uint32_t)idx * 48u));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const u64x2 v = r[k];
+    jw[2 * k] = v.x;
+    jw[2 * k + 1] = v.y;
+  }
+}
+
+// ENG: the launch also accumulates the energies of the new states (mode
+// carries kPhEnergy, and kPhEnergyK for K).  The items of both phases are
+// straight-line code (no per-item branches), so the compiler can issue the
+// loads of later items early.
+template <int L, bool FIELD, bool ENG>
+__global__ __launch_bounds__(SlabGeo<L>::NT) void k_slab(const JobPtrs *__restrict__ jt,
+                                                         const TaskDesc *__restrict__ tasks,
+                                                         int dir, int it, int mode) {
+  using G = SlabGeo<L>;
+  static_assert(G::PU * G::NT <= 2 * G::E * G::PL, "fold scratch within the colour-0 planes");
+  __shared__ uint64_t c0[2][G::E][G::PL];
+  __shared__ uint64_t c1s[G::C1W];
+  __shared__ int32_t redi[4 * 64];
+  const TaskDesc tk = ld_uniform(tasks + blockIdx.x);
+  if (tk.lat[1] >= (uint32_t)G::NP) return;   // padding task: the whole work-group
+  const JobPtrs jp = ld_uniform(jt + tk.job);
+  const LatDesc d = ld_uniform(jp.descs + tk.lat[0]);
+  const SG_GLOBAL uint64_t *src = to_global(dir ? jp.spins2 : jp.spins) + d.spin_off;
+  SG_GLOBAL uint64_t *dst = to_global(dir ? jp.spins : jp.spins2) + d.spin_off;
+  const SG_GLOBAL char *__restrict__ Js = (const SG_GLOBAL char *)(to_global(jp.Js) + 2 * d.j_off);
+  SG_GLOBAL int32_t *__restrict__ energy = to_global(jp.energy) + (size_t)d.e_idx * 128;
+  const int tid = threadIdx.x;
+  const int za = (int)tk.lat[1] * G::S;
+  if ((mode & kPhZero) && tk.lat[1] == 0 && tid < 128) energy[tid] = 0;
+
+  // colour-1 planes za - 2 ... za + S + 1 of both halves, staged in LDS
+  auto c1 = [&](int half, int e1, int col) -> uint64_t & {
+    return c1s[(half * (G::S + 4) + e1) * G::PL + col];
+  };
+  {
+    constexpr int W = G::C1W, NL = (W + G::NT - 1) / G::NT;
+    uint64_t v[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int i = tid + k * G::NT;
+      if (i < W) {
+        const int col = i % G::PL, r = i / G::PL;
+        const int half = r / (G::S + 4), e1 = r % (G::S + 4);
+        const int z = (za - 2 + e1 + half * G::LH + 2 * L) % L;
+        v[k] = src[G::NH + col + G::PL * z];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int i = tid + k * G::NT;
+      if (i < W) c1s[i] = v[k];
+    }
+  }
+
+  // per column: x offsets by site parity (XE[t] for parity (y + t) & 1,
+  // t = (z + c) & 1), y offsets
+  int cols[G::CPT], XE[G::CPT][2][2], dyp[G::CPT], dym[G::CPT];
+#pragma unroll
+  for (int i = 0; i < G::CPT; ++i) {
+    const int col = tid + i * G::NT;
+    cols[i] = col;
+    const int xh = col % G::LH, y = col / G::LH;
+    const int dxp = xh == G::LH - 1 ? -(G::LH - 1) : 1;
+    const int dxm = xh == 0 ? G::LH - 1 : -1;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const bool odd = ((y + t) & 1) != 0;
+      XE[i][t][0] = odd ? dxp : 0;
+      XE[i][t][1] = odd ? 0 : dxm;
+    }
+    dyp[i] = y == L - 1 ? -(L - 1) * G::LH : G::LH;
+    dym[i] = y == 0 ? (L - 1) * G::LH : -G::LH;
+  }
+  const ThrRec tr = ld_uniform(jp.thr + d.thr_idx);
+  const uint32_t n = ld_uniform(jp.js).sweep + (uint32_t)it;
+  const uint32_t sw = (kTagMC << 28) | (d.rt & 0x0FFFFFFFu);
+  __syncthreads();
+
+  // ---- phase 1: colour 0 on planes za - 1 ... za + S of both halves
+  uint32_t r23[G::S][G::CPT][2];
+#pragma unroll
+  for (int e = 0; e < G::E; ++e) {
+    const int pl = (za - 1 + e + L) % L;               // lower plane (uniform)
+    const int ph = (pl + G::LH) % L;
+    const int z0 = pl % G::LH;                         // the quad's plane
+    const bool slo = pl >= G::LH;                      // site index of the lower plane
+#pragma unroll
+    for (int i = 0; i < G::CPT; ++i) {
+      const int col = cols[i];
+      const u32x4 r = philox_dev((uint32_t)(col + G::PL * z0), n, d.g, sw, d.key0, d.key1);
+      if (e >= 1 && e <= G::S) {
+        r23[e - 1][i][0] = r.v[2];
+        r23[e - 1][i][1] = r.v[3];
+      }
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        const int z = half ? ph : pl;
+        const int h = col + G::PL * z;
+        const int t = z & 1;
+        uint64_t nv[6];
+        nv[0] = c1(half, e + 1, col + XE[i][t][0]);
+        nv[1] = c1(half, e + 1, col + XE[i][t][1]);
+        nv[2] = c1(half, e + 1, col + dyp[i]);
+        nv[3] = c1(half, e + 1, col + dym[i]);
+        nv[4] = c1(half, e + 2, col);
+        nv[5] = c1(half, e, col);
+        uint64_t jw[6];
+        site_record_free(Js, h, jw);
+        uint64_t sv = gload(src, h);
+        update_site<L, FIELD>(sv, nv, jw, (half ? !slo : slo) ? r.v[1] : r.v[0], tr);
+        c0[half][e][col] = sv;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 2: colour 1 on the owned planes; both colours leave
+  uint64_t AU[1][G::PU] = {}, AK[1][G::PK] = {};
+#pragma unroll
+  for (int e = 1; e <= G::S; ++e) {
+    const int pl = za - 1 + e;                         // < L/2: site 0 of its quad
+#pragma unroll
+    for (int i = 0; i < G::CPT; ++i) {
+      const int col = cols[i];
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        const int z = pl + half * G::LH;
+        const int h = col + G::PL * z;
+        const int t = (z + 1) & 1;
+        uint64_t nv[6];
+        nv[0] = c0[half][e][col + XE[i][t][0]];
+        nv[1] = c0[half][e][col + XE[i][t][1]];
+        nv[2] = c0[half][e][col + dyp[i]];
+        nv[3] = c0[half][e][col + dym[i]];
+        nv[4] = c0[half][e + 1][col];
+        nv[5] = c0[half][e - 1][col];
+        uint64_t jw[6];
+        site_record_free(Js + G::NH * 48, h, jw);
+        uint64_t sv = c1(half, e + 1, col);
+        update_site<L, FIELD>(sv, nv, jw, r23[e - 1][i][half], tr);
+        const uint64_t b = c0[half][e][col];
+        dst[G::NH + h] = sv;
+        dst[h] = b;
+        if (ENG) {
+          uint64_t u0, u1, u2;
+          bond_count(sv, nv, jw, u0, u1, u2);
+          vc_add3(AU[0], u0, u1, u2);
+          vc_add1(AK[0], sv);
+          vc_add1(AK[0], b);
+        }
+      }
+    }
+  }
+  if (ENG) {
+    uint64_t *fold = &c0[0][0][0];
+    __syncthreads();
+    int32_t U[1];
+    wg_count_lanes<G::PU, G::NT, 1>(AU, fold, redi, U);
+    if (tid < 64) atomicAdd((int32_t *)(energy + tid), U[0]);
+    if (mode & kPhEnergyK) {
+      __syncthreads();
+      int32_t K[1];
+      wg_count_lanes<G::PK, G::NT, 1>(AK, fold, redi, K);
+      if (tid < 64) atomicAdd((int32_t *)(energy + 64 + tid), K[0]);
+    }
+  }
+}
+
 // PT step of streamed lattices: a task is (pair, chunk of kPtWords words);
 // every work-group decides the 64 lanes itself (same energies, same Philox
 // words, so the same mask) and exchanges its chunk; chunk 0 counts.
@@ -1909,6 +2132,42 @@ hipError_t launch_phase(int L, bool field, const JobPtrs *jt, const TaskDesc *ta
     return hipGetLastError();
     SG_PHASE_SIZES(X)
 #undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+#define SG_SLAB_SIZES(X) X(32)
+
+bool slab_kernel_supported(int L) {
+  switch (L) {
+#define X(l) case l: return true;
+    SG_SLAB_SIZES(X)
+#undef X
+    default: return false;
+  }
+}
+
+hipError_t launch_slab(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
+                       int dir, int it, int mode, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  if (!(mode & kPhUpdate)) return hipErrorInvalidValue;   // sweeps only
+  const bool eng = (mode & (kPhEnergy | kPhEnergyK)) != 0;
+  switch (L) {
+#define SG_SLAB_LAUNCH(l, f, e)                                                            \
+  hipLaunchKernelGGL((k_slab<l, f, e>), dim3(ntasks), dim3(SlabGeo<l>::NT), 0, st, jt, tasks, \
+                     dir, it, mode)
+#define X(l)                                                                               \
+  case l:                                                                                  \
+    if (field) {                                                                           \
+      if (eng) SG_SLAB_LAUNCH(l, true, true); else SG_SLAB_LAUNCH(l, true, false);         \
+    } else {                                                                               \
+      if (eng) SG_SLAB_LAUNCH(l, false, true); else SG_SLAB_LAUNCH(l, false, false);       \
+    }                                                                                      \
+    return hipGetLastError();
+    SG_SLAB_SIZES(X)
+#undef X
+#undef SG_SLAB_LAUNCH
     default:
       return hipErrorInvalidValue;
   }

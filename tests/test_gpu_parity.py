@@ -85,10 +85,12 @@ def test_large_lattice_sweeps_match_oracle(L):
         np.testing.assert_array_equal(rec, orc.meas(i))
 
 
-@pytest.mark.parametrize("L", [12, 16, 18, 24, 32, 48])
+@pytest.mark.parametrize("L", [12, 16, 18, 24, 32, 40, 48])
 def test_streamed_kernel_matches_oracle(L):
     """One launch per colour over (lattice, chunk) tasks, PT in its own
-    launch, energies accumulated by the last colour launch."""
+    launch, energies accumulated by the last colour launch; at L = 32 pairs
+    of sweeps run as slab launches (one per sweep, double-buffered spins) and
+    odd leftovers as colour launches: segments of 2 and 1 sweeps here."""
     betas = geometric_betas(0.3, 2.0, 4)
     sim, orc = _pair(L, 80, 2, betas, pt_every=2, meas_every=3, kernel=sg.SG_KERNEL_STREAMED)
     sim.run(4)
@@ -101,10 +103,10 @@ def test_streamed_kernel_matches_oracle(L):
         np.testing.assert_array_equal(rec, orc.meas(i))
 
 
-@pytest.mark.parametrize("H", [0.25, 1.5])
-def test_streamed_kernel_field_matches_oracle(H):
+@pytest.mark.parametrize("L,H,pt", [(20, 0.25, 1), (20, 1.5, 1), (32, 0.5, 2), (48, 0.25, 4)])
+def test_streamed_kernel_field_matches_oracle(L, H, pt):
     betas = geometric_betas(0.3, 2.0, 3)
-    sim, orc = _pair(20, 64, 2, betas, H=H, pt_every=1, meas_every=3,
+    sim, orc = _pair(L, 64, 2, betas, H=H, pt_every=pt, meas_every=3,
                      kernel=sg.SG_KERNEL_STREAMED)
     sim.run(5)
     orc.run(5)
@@ -112,12 +114,13 @@ def test_streamed_kernel_field_matches_oracle(H):
     np.testing.assert_array_equal(sim.records[0][1], orc.meas(0))
 
 
-def test_resident_and_streamed_kernels_agree_in_both_forms():
+@pytest.mark.parametrize("L", [20, 32])
+def test_resident_and_streamed_kernels_agree_in_both_forms(L):
     betas = geometric_betas(0.2, 2.0, 6)
     runs = []
     for kernel in (sg.SG_KERNEL_RESIDENT, sg.SG_KERNEL_STREAMED):
         for form in (sg.SG_FORM_PLAIN, sg.SG_FORM_BATCHED):
-            s = sg.Sim(L=20, n_samples=128, n_replicas=2, betas=betas, pt_every=4, meas_every=8,
+            s = sg.Sim(L=L, n_samples=128, n_replicas=2, betas=betas, pt_every=4, meas_every=8,
                        form=form, kernel=kernel)
             s.init_lattice()
             s.run(36)          # the batched form replays its captured intervals
