@@ -85,6 +85,33 @@ def test_large_lattice_sweeps_match_oracle(L):
         np.testing.assert_array_equal(rec, orc.meas(i))
 
 
+@pytest.mark.parametrize("case", [
+    # the reference's CPU case: L = 8, 32 samples (half a lane group), 16
+    # temperatures geometric in [0.5, 2], PT every sweep
+    dict(L=8, n=32, R=2, T=(0.5, 2.0, 16), pt=1, meas=4),
+    dict(L=12, n=64, R=1, T=(0.2, 2.0, 5), pt=2, meas=3),            # one replica
+    dict(L=16, n=64, R=2, T=(0.5, 0.5, 1), pt=4, meas=4),            # one temperature: no pairs
+    dict(L=20, n=64, R=2, T=(0.2, 2.0, 4), pt=0, meas=0),            # no PT, no measurement
+    dict(L=24, n=130, R=2, T=(0.3, 2.0, 3), pt=2, meas=3,            # three groups, k_one,
+         form=sg.SG_FORM_BATCHED, kernel=sg.SG_KERNEL_RESIDENT),      # captured intervals
+    dict(L=32, n=64, R=1, T=(0.3, 2.0, 2), pt=2, meas=0,             # slab sweeps, one replica
+         kernel=sg.SG_KERNEL_STREAMED),
+], ids=["config1_L8", "R1", "T1", "noPT", "L24_3groups_batched", "L32_slab_R1"])
+def test_edge_shapes_match_oracle(case):
+    tmin, tmax, nt = case["T"]
+    betas = geometric_betas(tmin, tmax, nt)
+    sim, orc = _pair(case["L"], case["n"], case["R"], betas, pt_every=case["pt"],
+                     meas_every=case["meas"], form=case.get("form", sg.SG_FORM_PLAIN),
+                     kernel=case.get("kernel", sg.SG_KERNEL_AUTO))
+    for n in (5, 4, 8):
+        sim.run(n)
+        orc.run(n)
+        _assert_same(sim, orc)
+    assert len(sim.records) == orc.n_meas()
+    for i, (_, rec) in enumerate(sim.records):
+        np.testing.assert_array_equal(rec, orc.meas(i))
+
+
 @pytest.mark.parametrize("L", [12, 16, 18, 24, 32, 40, 48])
 def test_streamed_kernel_matches_oracle(L):
     """One launch per colour over (lattice, chunk) tasks, PT in its own
