@@ -427,8 +427,27 @@ struct sg_handle {
           }
         // within a group, pairs (twice the work) first, then the single
         // lattices, so that the dispatcher's greedy fill ends evenly
-        std::stable_partition(tv.begin() + task_off[par][g], tv.end(),
-                              [](const TaskDesc &k) { return k.lat[1] != 0xFFFFFFFFu; });
+        auto first_single = std::stable_partition(
+            tv.begin() + task_off[par][g], tv.end(),
+            [](const TaskDesc &k) { return k.lat[1] != 0xFFFFFFFFu; });
+        // two-lattice kernels: the lattices left without a partner at this
+        // parity (slots 0 and T-1 at odd parity) share a work-group, two by
+        // two, with no PT between them (they share the group's couplings)
+        if (!single) {
+          std::vector<TaskDesc> singles(first_single, tv.end());
+          tv.erase(first_single, tv.end());
+          std::vector<TaskDesc> rest;
+          for (size_t i = 0; i < singles.size(); i += 2) {
+            TaskDesc k = singles[i];
+            if (i + 1 < singles.size()) {
+              k.lat[1] = singles[i + 1].lat[0];
+              tv.push_back(k);
+            } else {
+              rest.push_back(k);
+            }
+          }
+          tv.insert(tv.end(), rest.begin(), rest.end());
+        }
       }
       if (form == SG_FORM_BATCHED)  // one launch: all pairs of all groups first
         std::stable_partition(tv.begin(), tv.end(),

@@ -11,8 +11,9 @@ namespace sg {
 // memory) above
 bool sweep_kernel_supported(int L);
 
-// One work-group per task (one lattice, or the lattices of slots t and t+1
-// of one group and replica); a task's buffers are those of its job,
+// One work-group per task (one lattice, or two lattices of one group: a PT
+// pair of slots t and t+1 of one replica, or two lattices left without a
+// partner at this PT parity); a task's buffers are those of its job,
 // jt[task.job], so one launch may cover many jobs of one L.  With do_pt, the
 // task's PT pair is decided and exchanged first; then nsweeps sweeps;
 // want_energy = 1 writes the per-lane U of the final states to the energy

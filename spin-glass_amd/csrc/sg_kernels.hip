@@ -103,6 +103,32 @@ __device__ __forceinline__ SG_GLOBAL T *to_global(T *p) {
   return (SG_GLOBAL T *)p;
 }
 
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
+// A 32-bit byte offset the compiler cannot see through (a pure asm, free to
+// move): loads from a uniform base at such an offset take the SGPR-base form
+// and no 64-bit address per item is formed or kept.
+__device__ __forceinline__ uint32_t opaque(uint32_t v) {
+  asm("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ uint64_t gload(const SG_GLOBAL uint64_t *base, int idx) {
+  return *(const SG_GLOBAL uint64_t *)((const SG_GLOBAL char *)base + opaque((uint32_t)idx * 8u));
+}
+
+// site record idx of Jsb without the offset barrier: the compiler may issue
+// these loads as early as it likes (the items that use them are independent)
+__device__ __forceinline__ void site_record_free(const SG_GLOBAL char *__restrict__ Jsb, int idx,
+                                                 uint64_t (&jw)[6]) {
+  const SG_GLOBAL u64x2 *r = (const SG_GLOBAL u64x2 *)(Jsb + opaque((uint32_t)idx * 48u));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const u64x2 v = r[k];
+    jw[2 * k] = v.x;
+    jw[2 * k + 1] = v.y;
+  }
+}
+
 template <int L>
 struct Geo {
   static constexpr int N = L * L * L;
@@ -475,7 +501,6 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   const TaskDesc tk = ld_uniform(tasks + blockIdx.x);
   const JobPtrs jp = ld_uniform(jt + tk.job);
   SG_GLOBAL uint64_t *__restrict__ spins = to_global(jp.spins);
-  const SG_GLOBAL uint64_t *__restrict__ Jg = to_global(jp.J);
   SG_GLOBAL int32_t *__restrict__ energy = to_global(jp.energy);
   unsigned long long *__restrict__ counters = jp.counters;
   const bool two = tk.lat[1] != kNone;
@@ -519,7 +544,9 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
 
   // neighbour indices within the opposite colour's half (y, z: same for both
   // colours; x: per colour) and the 6 couplings of each owned site
-  const SG_GLOBAL uint64_t *Jx = Jg + d0.j_off, *Jy = Jx + G::N, *Jz = Jy + G::N;
+  // (couplings from the site records: three 16-byte loads per site,
+  // contiguous across the wave)
+  const SG_GLOBAL char *Js = (const SG_GLOBAL char *)(to_global(jp.Js) + 2 * d0.j_off);
   int yz[2][4], xi[2][2][2];
   uint64_t jw[2][2][6];
   if (active) {
@@ -539,13 +566,7 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
         const int xp = (x + 1) % L, xm = (x + L - 1) % L;
         xi[q][c][0] = (xp >> 1) + G::LH * (y + L * z);
         xi[q][c][1] = (xm >> 1) + G::LH * (y + L * z);
-        const int s = x + L * (y + L * z);
-        jw[q][c][0] = Jx[s];
-        jw[q][c][1] = Jx[xm + L * (y + L * z)];
-        jw[q][c][2] = Jy[s];
-        jw[q][c][3] = Jy[x + L * (ym + L * z)];
-        jw[q][c][4] = Jz[s];
-        jw[q][c][5] = Jz[x + L * (y + L * zm)];
+        site_record_free(Js, c * G::NH + h, jw[q][c]);
       }
     }
   }
@@ -913,7 +934,6 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("" ::: "memory");
 }
 
-typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
 // The six bond words of record idx of the record array at Jsb (a group's
 // block, or its colour-c half): a uniform base and a 32-bit byte offset, so
@@ -1753,29 +1773,7 @@ struct SlabGeo {
   static_assert((2 * E * PL + C1W) * 8 <= 150 * 1024, "both colours' planes in LDS");
 };
 
-// A 32-bit byte offset the compiler cannot see through (a pure asm, free to
-// move): loads from a uniform base at such an offset take the SGPR-base form
-// and no 64-bit address per item is formed or kept.
-__device__ __forceinline__ uint32_t opaque(uint32_t v) {
-  asm("" : "+v"(v));
-  return v;
-}
-__device__ __forceinline__ uint64_t gload(const SG_GLOBAL uint64_t *base, int idx) {
-  return *(const SG_GLOBAL uint64_t *)((const SG_GLOBAL char *)base + opaque((uint32_t)idx * 8u));
-}
 
-// site record idx of Jsb without the offset barrier: the compiler may issue
-// these loads as early as it likes (k_slab's items are independent)
-__device__ __forceinline__ void site_record_free(const SG_GLOBAL char *__restrict__ Jsb, int idx,
-                                                 uint64_t (&jw)[6]) {
-  const SG_GLOBAL u64x2 *r = (const SG_GLOBAL u64x2 *)(Jsb + opaque((uint32_t)idx * 48u));
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const u64x2 v = r[k];
-    jw[2 * k] = v.x;
-    jw[2 * k + 1] = v.y;
-  }
-}
 
 // ENG: the launch also accumulates the energies of the new states (mode
 // carries kPhEnergy, and kPhEnergyK for K).  The items of both phases are
