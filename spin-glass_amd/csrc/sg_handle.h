@@ -275,7 +275,7 @@ struct sg_handle {
 
   // batched form: one PT interval (sweep launch + counter advance) per
   // (parity, length, energy) captured once as a hipGraph and replayed
-  struct GraphKey { int parity, seg, energy; hipGraphExec_t exec; };
+  struct GraphKey { int parity, seg, energy, count; hipGraphExec_t exec; };
   std::vector<GraphKey> graphs;
 
   ~sg_handle() { release(); }
@@ -757,23 +757,37 @@ struct sg_handle {
     if (seg > 0) HIPCHK(launch_advance(d_self, 1, seg, st));
   }
 
-  // Batched form: a full PT interval is one hipGraph per (parity, length,
-  // energy), replayed while the device sweep counter advances.
-  void run_graph_interval(int par, int seg, int want_energy) {
+  // Batched form: `count` consecutive full PT intervals (parities par,
+  // 1 - par, ...) are one hipGraph per (parity, length, energy, count),
+  // replayed while the device sweep counter advances; fewer graph launches,
+  // fewer gaps between them.
+  void run_graph_interval(int par, int seg, int want_energy, int count = 1) {
     hipGraphExec_t exec = nullptr;
     for (auto &k : graphs)
-      if (k.parity == par && k.seg == seg && k.energy == want_energy) exec = k.exec;
+      if (k.parity == par && k.seg == seg && k.energy == want_energy && k.count == count)
+        exec = k.exec;
     if (!exec) {
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      launch_interval(par, 1, seg, want_energy);
+      for (int i = 0; i < count; ++i) launch_interval((par + i) & 1, 1, seg, want_energy);
       HIPCHK(hipStreamEndCapture(st, &g));
       hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
       hipGraphDestroy(g);
       HIPCHK(e);
-      graphs.push_back(GraphKey{par, seg, want_energy, exec});
+      graphs.push_back(GraphKey{par, seg, want_energy, count, exec});
     }
     HIPCHK(hipGraphLaunch(exec, st));
+  }
+
+  // full PT intervals from sweep n that one graph may replay: up to 4, all
+  // before the next measurement and within the remaining sweeps
+  int graph_run_length(int64_t n, int64_t remaining) const {
+    int64_t room = remaining / pt_every;
+    if (meas_every > 0) {
+      const int64_t m = n - n % meas_every + meas_every;   // next measurement sweep
+      room = std::min<int64_t>(room, (m - n) / pt_every);
+    }
+    return room >= 4 ? 4 : room >= 2 ? 2 : 1;
   }
 
   void run(int64_t n_sweeps, sg_obs_sink sink, void *user) {
@@ -794,20 +808,24 @@ struct sg_handle {
       if (pt_now && !energy_valid) launch_energy_all(energy_mode());
       const int par = pt_now ? (int)((n / pt_every) & 1) : 0;
       const int emode = want_energy ? energy_mode() : 0;
+      int count = 1;   // PT intervals run by this step
       if (form == SG_FORM_BATCHED && pt_now && seg == pt_every && !profiling) {
-        run_graph_interval(par, (int)seg, emode);
+        count = graph_run_length(n, remaining);
+        run_graph_interval(par, (int)seg, emode, count);
       } else {
         launch_interval(par, pt_now ? 1 : 0, (int)seg, emode);
       }
-      if (pt_now)
-        for (int t = par; t + 1 < T; t += 2) swap_att[t] += (uint64_t)G * R * 64;
-      sweep = next;
+      for (int i = 0; i < count; ++i) {
+        if (pt_now)
+          for (int t = (par + i) & 1; t + 1 < T; t += 2) swap_att[t] += (uint64_t)G * R * 64;
+        flips += (double)seg * N * 64.0 * G * R * T;
+      }
+      sweep = n + count * seg;
       energy_valid = want_energy;
-      flips += (double)seg * N * 64.0 * G * R * T;
-      remaining -= seg;
+      remaining -= count * seg;
       // records of earlier measurements whose copies have landed
       if (!rec_queue.empty()) drain_records(sink, user, 0);
-      if (meas_every > 0 && next % meas_every == 0) measure(sink, user);
+      if (meas_every > 0 && sweep % meas_every == 0) measure(sink, user);
     }
     drain_records(sink, user, -1);
     HIPCHK(hipEventRecord(ev1, st));
