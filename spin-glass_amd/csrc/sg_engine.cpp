@@ -310,6 +310,17 @@ struct sg_batch {
           out.h[out.n++] = t;
         }
       }
+    if (pairs_first) {
+      // resident tasks: each coupling group (job, group) on one XCD
+      std::vector<uint32_t> first(c.jobs.size() + 1, 0);   // class-wide group numbering
+      for (size_t ji = 0; ji < c.jobs.size(); ++ji) first[ji + 1] = first[ji] + c.jobs[ji]->h->G;
+      std::vector<TaskDesc> tv(out.h, out.h + out.n);
+      xcd_interleave(tv, [&](const TaskDesc &t) {
+        const sg_handle &h = *c.jobs[t.job]->h;
+        return first[t.job] + (uint32_t)(t.lat[0] / ((size_t)h.R * h.T));
+      });
+      std::copy(tv.begin(), tv.end(), out.h);
+    }
   }
 
   static const std::vector<TaskDesc> &pick_tasks(sg_handle &h, int par) { return h.h_tasks[par]; }

@@ -74,6 +74,34 @@ inline std::vector<TaskDesc> xcd_order(const std::vector<TaskDesc> &in, uint32_t
   return out;
 }
 
+// Resident tasks of one launch reordered so that all tasks with the same
+// coupling group (key) run on one XCD: queue x collects the tasks of keys
+// with key % kXcds == x, in order, and the queues are interleaved (work-group
+// i goes to XCD i % kXcds).  The group's couplings are then fetched into one
+// XCD's L2 instead of every one.  The two-lattice tasks stay ahead of the
+// single ones (they are twice the work).  Unequal queues leave the tail in
+// plain order.
+template <class KeyFn>
+inline void xcd_interleave(std::vector<TaskDesc> &tv, KeyFn key) {
+  std::vector<TaskDesc> out;
+  out.reserve(tv.size());
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<std::vector<TaskDesc>> q(kXcds);
+    for (const TaskDesc &t : tv)
+      if ((t.lat[1] != 0xFFFFFFFFu) == (pass == 0)) q[key(t) % kXcds].push_back(t);
+    std::vector<size_t> pos(kXcds, 0);
+    for (bool any = true; any;) {
+      any = false;
+      for (int x = 0; x < kXcds; ++x)
+        if (pos[x] < q[x].size()) {
+          out.push_back(q[x][pos[x]++]);
+          any = true;
+        }
+    }
+  }
+  tv.swap(out);
+}
+
 struct Fail {
   sg_status code;
   std::string msg;
@@ -449,9 +477,12 @@ struct sg_handle {
           tv.insert(tv.end(), rest.begin(), rest.end());
         }
       }
-      if (form == SG_FORM_BATCHED)  // one launch: all pairs of all groups first
-        std::stable_partition(tv.begin(), tv.end(),
-                              [](const TaskDesc &k) { return k.lat[1] != 0xFFFFFFFFu; });
+      if (form == SG_FORM_BATCHED) {  // one launch: all pairs of all groups first
+        const size_t per_group = (size_t)R * T;
+        xcd_interleave(tv, [per_group](const TaskDesc &k) {
+          return (uint32_t)(k.lat[0] / per_group);
+        });
+      }
       task_off[par][G] = (int)tv.size();
       h_tasks[par] = tv;
       dev_alloc(&d_tasks[par], tv.size());
