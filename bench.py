@@ -48,11 +48,11 @@ W_ALG = 1.22          # int32 VALU ops per spin-flip attempt (algorithmic)
 # 16-lane-per-SIMD figure (39.3) understates the hardware by 2x.
 VALU_PEAK_TOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_block<16,false> launch of the l16 workload (768 pair tasks,
+# HBM bytes per k_block<16,false> launch of the l16 workload (768 two-lattice tasks,
 # 16 sweeps): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of that kernel, each
 # scaled by what the counter reads for a device copy of known size (the
 # --copy-probe run); profiles/k_block_L16_batched_2048_traffic.txt.
-TRAFFIC_L16 = 131.8e6
+TRAFFIC_L16 = 109.2e6
 
 
 def make(samples, form):
