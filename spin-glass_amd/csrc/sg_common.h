@@ -171,7 +171,8 @@ struct TaskDesc {
 struct MeasDesc {
   uint64_t spin_off;      // replica 0 lattice
   uint64_t rho_stride;    // words between replicas
-  uint64_t j_off;
+  uint64_t e_idx;         // energy slot of replica 0 (U, K of the current states)
+  uint64_t e_stride;      // energy slots between replicas
   uint64_t out_off;       // int32 offset of the 64 records
   uint64_t acc_off;       // ObsAcc index of lane 0
   uint64_t cs_off;        // double offset of cos[L], sin[L] of 2 pi c / L

@@ -374,11 +374,13 @@ struct sg_batch {
   // ------------------------------------------------------------ one step
   // After each PT interval: host counters, measurements, and the final
   // copies of jobs that reached their sweep count (which leave the class).
-  void after_interval(Class &c) {
+  void after_interval(Class &c, int emode) {
     const int pe = bp.pt_every;
     bool left = false;
     for (Job *jp : c.jobs) {
       sg_handle &h = *jp->h;
+      h.energy_valid = true;
+      h.energy_k_valid = emode >= 2;
       const int par = (int)((h.sweep / pe) & 1);
       for (int t = par; t + 1 < h.T; t += 2) h.swap_att[t] += (uint64_t)h.G * h.R * 64;
       h.sweep += pe;
@@ -404,7 +406,12 @@ struct sg_batch {
   void run_class(Class &c) {
     const int pe = bp.pt_every;
     for (int i = 0; i < bp.slice_pt_steps && !c.jobs.empty(); ++i) {
-      const int emode = c.field ? 2 : 1;
+      // U (and K at H != 0) for the next PT step; U and K when a job of the
+      // class measures after this interval
+      bool meas = false;
+      for (Job *jp : c.jobs)
+        meas = meas || (jp->h->meas_every > 0 && (jp->h->sweep + pe) % jp->h->meas_every == 0);
+      const int emode = c.field || meas ? 2 : 1;
       if (bp.form == SG_FORM_BATCHED) {
         if (c.dirty) build(c);
         const int k = (int)(c.steps_since_build & 1);
@@ -445,7 +452,7 @@ struct sg_batch {
           h.launch_interval((int)((h.sweep / pe) & 1), 1, pe, emode);
         }
       }
-      after_interval(c);
+      after_interval(c, emode);
     }
   }
 

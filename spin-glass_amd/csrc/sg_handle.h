@@ -253,6 +253,7 @@ struct sg_handle {
   size_t rec_count = 0;
   std::vector<uint64_t> swap_att; // [T-1]
   bool initialized = false, energy_valid = false;
+  bool energy_k_valid = false;   // the energy slots hold U and K of the current states
   int64_t sweep = 0, n_meas = 0;
   double kernel_ms = 0, flips = 0;
 
@@ -496,7 +497,8 @@ struct sg_handle {
         MeasDesc &d = md[(size_t)g * T + t];
         d.spin_off = lat_index(g, 0, t) * (uint64_t)N;
         d.rho_stride = (uint64_t)T * N;
-        d.j_off = (uint64_t)g * 3 * N;
+        d.e_idx = lat_index(g, 0, t);
+        d.e_stride = (uint64_t)T;
         d.out_off = ((uint64_t)g * T + t) * 64 * nfield;
         d.acc_off = ((uint64_t)g * T + t) * 64;
         d.cs_off = 0;
@@ -670,7 +672,7 @@ struct sg_handle {
     HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * std::max(T - 1, 1), st));
     std::fill(swap_att.begin(), swap_att.end(), 0);
     reset_counters();
-    energy_valid = false;
+    energy_valid = energy_k_valid = false;
     initialized = true;
   }
 
@@ -698,7 +700,7 @@ struct sg_handle {
       for (int s = 0; s < N; ++s) raw[l * N + perm[s]] = in[l * N + s];
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipMemcpy(d_spins, raw.data(), sizeof(uint64_t) * raw.size(), hipMemcpyHostToDevice));
-    energy_valid = false;
+    energy_valid = energy_k_valid = false;
   }
 
   // ------------------------------------------------------------ run loop
@@ -838,21 +840,25 @@ struct sg_handle {
       const bool pt_now = pt && n % pt_every == 0;
       if (pt_now && !energy_valid) launch_energy_all(energy_mode());
       const int par = pt_now ? (int)((n / pt_every) & 1) : 0;
-      const int emode = want_energy ? energy_mode() : 0;
       int count = 1;   // PT intervals run by this step
-      if (form == SG_FORM_BATCHED && pt_now && seg == pt_every && !profiling) {
-        count = graph_run_length(n, remaining);
+      const bool graphed = form == SG_FORM_BATCHED && pt_now && seg == pt_every && !profiling;
+      if (graphed) count = graph_run_length(n, remaining);
+      // energies of the final states: U (and K at H != 0) for the next PT
+      // step, U and K when a measurement follows
+      const bool meas_end = meas_every > 0 && (n + count * seg) % meas_every == 0;
+      const int emode = meas_end ? 2 : want_energy ? energy_mode() : 0;
+      if (graphed)
         run_graph_interval(par, (int)seg, emode, count);
-      } else {
+      else
         launch_interval(par, pt_now ? 1 : 0, (int)seg, emode);
-      }
       for (int i = 0; i < count; ++i) {
         if (pt_now)
           for (int t = (par + i) & 1; t + 1 < T; t += 2) swap_att[t] += (uint64_t)G * R * 64;
         flips += (double)seg * N * 64.0 * G * R * T;
       }
       sweep = n + count * seg;
-      energy_valid = want_energy;
+      energy_valid = emode > 0;
+      energy_k_valid = emode >= 2;
       remaining -= count * seg;
       // records of earlier measurements whose copies have landed
       if (!rec_queue.empty()) drain_records(sink, user, 0);
@@ -872,8 +878,14 @@ struct sg_handle {
   void measure(sg_obs_sink sink, void *user) {
     const bool want = sink != nullptr;
     if (want && !d_rec) throw Fail{SG_ERR_STATE, "this handle delivers no records"};
+    // U and K of the current states come from the energy slots (the sweep
+    // launch ending here fills them when it knows a measurement follows)
+    if (!energy_k_valid) {
+      launch_energy_all(2);
+      energy_valid = energy_k_valid = true;
+    }
     timed(2, 0.0, [&] {
-      HIPCHK(launch_measure(L, R, d_spins, d_J, d_meas, G * T, d_rec, nfield, d_acc, d_cs,
+      HIPCHK(launch_measure(L, R, d_spins, d_energy, d_meas, G * T, d_rec, nfield, d_acc, d_cs,
                             want ? 1 : 0, st));
     });
     ++n_meas;
@@ -1037,7 +1049,7 @@ struct sg_handle {
     sweep = h.sweep;
     n_meas = h.n_meas;
     reset_counters();
-    energy_valid = false;
+    energy_valid = energy_k_valid = false;
   }
 };
 

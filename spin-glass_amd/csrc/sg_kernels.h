@@ -31,7 +31,7 @@ hipError_t launch_single(int L, bool field, const JobPtrs *jt, const TaskDesc *t
 
 // Measures every (group, temperature) of md: folds the measurement into the
 // ObsAcc sums and, with write_records, writes the int32 records to out.
-hipError_t launch_measure(int L, int R, const uint64_t *spins, const uint64_t *J,
+hipError_t launch_measure(int L, int R, const uint64_t *spins, const int32_t *energy,
                           const MeasDesc *md, int nmd, int32_t *out, int nfield, ObsAcc *acc,
                           const double *cs, int write_records, hipStream_t st);
 

@@ -1432,73 +1432,77 @@ __global__ __launch_bounds__(OneGeo<L>::NT) void k_one(const JobPtrs *__restrict
 }
 
 // ---------------------------------------------------------- measure kernel
-// One work-group (4 waves) per (group, temperature).  Lane l of every wave
-// is sample l; each wave walks a quarter of the (y, z) rows with wave-uniform
-// (scalar) loads and counts, per lane, U and K of every replica and the
-// overlap W with its x-, y- and z-plane profiles.  Reference counterpart:
-// compute_q (eamsim.cu:451-511) plus the E, M scan of perform_PT_shuffle.
+// One work-group (4 waves) per (group, temperature); lane l of every wave is
+// sample l.  U and K of every replica are the per-lane energy slots, which
+// the sweep launch ending at a measurement fills (U and K).  The overlap
+// w = s1 XOR s2 of replicas 0 and 1 is staged in LDS a chunk of z-planes at
+// a time with coalesced loads; each wave then walks a quarter of the chunk's
+// (y, z) rows reading the words as LDS broadcasts, and each lane counts its
+// own bit into W and the x-, y- and z-plane profiles.  Reference
+// counterpart: compute_q (eamsim.cu:451-511) plus the E, M scan of
+// perform_PT_shuffle.
+template <int L>
+struct MeasGeo {
+  static constexpr int N = L * L * L;
+  static constexpr int NH = N / 2;
+  static constexpr int LH = L / 2;
+  static constexpr int PL = LH * L;                   // words per colour plane
+  // z-planes per chunk: the largest divisor of L whose staging fits 48 KiB
+  static constexpr int zc(int z) { return z <= 1 ? 1 : (L % z == 0 && 16 * PL * z <= 49152 ? z : zc(z - 1)); }
+  static constexpr int ZC = zc(L);
+};
+
 template <int L>
 __global__ __launch_bounds__(256) void k_measure(const uint64_t *__restrict__ spins,
-                                                 const uint64_t *__restrict__ Jg,
+                                                 const int32_t *__restrict__ energy,
                                                  const MeasDesc *__restrict__ md, int R,
                                                  int32_t *__restrict__ out, int nfield,
                                                  ObsAcc *__restrict__ acc,
                                                  const double *__restrict__ cs,
                                                  int write_records) {
-  constexpr int N = L * L * L;
+  using G = MeasGeo<L>;
+  constexpr int N = G::N;
+  __shared__ uint64_t wbuf[2][G::ZC][G::PL];
   __shared__ int32_t sx[L][64], sy[L][64], sz[L][64];
-  __shared__ int32_t sU[8][64], sK[8][64], sW[64];
+  __shared__ int32_t sW[64];
   const MeasDesc d = md[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  for (int i = tid; i < L * 64; i += 256) {
-    (&sx[0][0])[i] = 0; (&sy[0][0])[i] = 0; (&sz[0][0])[i] = 0;
-  }
-  for (int i = tid; i < 8 * 64; i += 256) { (&sU[0][0])[i] = 0; (&sK[0][0])[i] = 0; }
-  if (tid < 64) sW[tid] = 0;
-  __syncthreads();
-  const uint64_t *Jx = Jg + d.j_off, *Jy = Jx + N, *Jz = Jy + N;
-  for (int rho = 0; rho < R; ++rho) {
-    const uint64_t *S = spins + d.spin_off + (uint64_t)rho * d.rho_stride;
-    int32_t u = 0, k = 0;
-    for (int row = wave; row < L * L; row += 4) {
-      const int y = row % L, z = row / L;
-      const int yp = (y + 1) % L, zp = (z + 1) % L;
-#pragma unroll 4
-      for (int x = 0; x < L; ++x) {
-        const int xp = (x + 1) % L;
-        const int sn = x + L * (y + L * z);
-        const uint64_t s = S[cb_index<L>(x, y, z)];
-        const uint64_t bx = s ^ S[cb_index<L>(xp, y, z)] ^ Jx[sn];
-        const uint64_t by = s ^ S[cb_index<L>(x, yp, z)] ^ Jy[sn];
-        const uint64_t bz = s ^ S[cb_index<L>(x, y, zp)] ^ Jz[sn];
-        u += (int32_t)((bx >> lane) & 1) + (int32_t)((by >> lane) & 1) +
-             (int32_t)((bz >> lane) & 1);
-        k += (int32_t)((s >> lane) & 1);
-      }
-    }
-    atomicAdd(&sU[rho][lane], u);
-    atomicAdd(&sK[rho][lane], k);
-  }
   if (R >= 2) {
+    for (int i = tid; i < L * 64; i += 256) {
+      (&sx[0][0])[i] = 0;
+      (&sy[0][0])[i] = 0;
+      (&sz[0][0])[i] = 0;
+    }
+    if (tid < 64) sW[tid] = 0;
     const uint64_t *S0 = spins + d.spin_off, *S1 = S0 + d.rho_stride;
     int32_t wx[L];
 #pragma unroll
     for (int x = 0; x < L; ++x) wx[x] = 0;
     int32_t wt = 0;
-    for (int row = wave; row < L * L; row += 4) {
-      const int y = row % L, z = row / L;
-      int32_t rs = 0;
-#pragma unroll
-      for (int x = 0; x < L; ++x) {
-        const int i = cb_index<L>(x, y, z);
-        const int32_t wb = (int32_t)(((S0[i] ^ S1[i]) >> lane) & 1);
-        wx[x] += wb;
-        rs += wb;
+    for (int z0 = 0; z0 < L; z0 += G::ZC) {
+      __syncthreads();   // the previous chunk has been read
+      constexpr int W = 2 * G::ZC * G::PL;
+      for (int i = tid; i < W; i += 256) {
+        const int c = i / (G::ZC * G::PL), r = i % (G::ZC * G::PL);
+        const int k = c * G::NH + z0 * G::PL + r;     // word: colour c, plane z0 + r / PL
+        (&wbuf[0][0][0])[i] = S0[k] ^ S1[k];
       }
-      atomicAdd(&sy[y][lane], rs);
-      atomicAdd(&sz[z][lane], rs);
-      wt += rs;
+      __syncthreads();
+      for (int row = wave; row < G::ZC * L; row += 4) {
+        const int zz = row / L, y = row % L, z = z0 + zz;
+        int32_t rs = 0;
+#pragma unroll
+        for (int x = 0; x < L; ++x) {
+          const int c = (x + y + z) & 1;
+          const int32_t wb = (int32_t)((wbuf[c][zz][(x >> 1) + G::LH * y] >> lane) & 1);
+          wx[x] += wb;
+          rs += wb;
+        }
+        atomicAdd(&sy[y][lane], rs);
+        atomicAdd(&sz[z][lane], rs);
+        wt += rs;
+      }
     }
 #pragma unroll
     for (int x = 0; x < L; ++x) atomicAdd(&sx[x][lane], wx[x]);
@@ -1509,8 +1513,9 @@ __global__ __launch_bounds__(256) void k_measure(const uint64_t *__restrict__ sp
     // fold this measurement into the lane's running sums (one owner per lane)
     ObsAcc a = acc[d.acc_off + tid];
     for (int rho = 0; rho < R; ++rho) {
-      a.sU += sU[rho][tid];
-      a.sK += sK[rho][tid];
+      const int32_t *e = energy + (size_t)(d.e_idx + (uint64_t)rho * d.e_stride) * 128;
+      a.sU += e[tid];
+      a.sK += e[64 + tid];
     }
     if (R >= 2) {
 #pragma clang fp contract(off)
@@ -1546,10 +1551,12 @@ __global__ __launch_bounds__(256) void k_measure(const uint64_t *__restrict__ sp
   for (int i = tid; i < 64 * nfield; i += 256) {
     const int l = i / nfield, f = i % nfield;
     int32_t v;
-    if (f < R) v = sU[f][l];
-    else if (f < 2 * R) v = sK[f - R][l];
-    else if (f == 2 * R) v = R >= 2 ? sW[l] : 0;
-    else {
+    if (f < 2 * R) {
+      const int rho = f < R ? f : f - R;
+      v = energy[(size_t)(d.e_idx + (uint64_t)rho * d.e_stride) * 128 + (f < R ? 0 : 64) + l];
+    } else if (f == 2 * R) {
+      v = R >= 2 ? sW[l] : 0;
+    } else {
       const int g = f - 2 * R - 1;
       const int dir = g / L, c = g % L;
       v = R < 2 ? 0 : (dir == 0 ? sx[c][l] : dir == 1 ? sy[c][l] : sz[c][l]);
@@ -2058,7 +2065,7 @@ hipError_t launch_single(int L, bool field, const JobPtrs *jt, const TaskDesc *t
   }
 }
 
-hipError_t launch_measure(int L, int R, const uint64_t *spins, const uint64_t *J,
+hipError_t launch_measure(int L, int R, const uint64_t *spins, const int32_t *energy,
                           const MeasDesc *md, int nmd, int32_t *out, int nfield, ObsAcc *acc,
                           const double *cs, int write_records, hipStream_t st) {
   if (nmd <= 0) return hipSuccess;
@@ -2066,7 +2073,7 @@ hipError_t launch_measure(int L, int R, const uint64_t *spins, const uint64_t *J
   switch (L) {
 #define X(l)                                                                            \
   case l:                                                                               \
-    hipLaunchKernelGGL(k_measure<l>, dim3(nmd), dim3(256), 0, st, spins, J, md, R, out, \
+    hipLaunchKernelGGL(k_measure<l>, dim3(nmd), dim3(256), 0, st, spins, energy, md, R, out, \
                        nfield, acc, cs, write_records);                                 \
     return hipGetLastError();
     SG_ALL_SIZES(X)

@@ -189,7 +189,7 @@ sg_status sg_set_couplings(sg_handle *h, const uint64_t *in) {
     HIPCHK(hipMemcpy(h->d_J, in, sizeof(uint64_t) * (size_t)h->G * 3 * h->N,
                      hipMemcpyHostToDevice));
     h->build_jsite();
-    h->energy_valid = false;
+    h->energy_valid = h->energy_k_valid = false;
   });
 }
 
@@ -198,7 +198,7 @@ sg_status sg_get_energies(sg_handle *h, int32_t *U, int32_t *K) {
   return guarded(h, [&] {
     // always a fresh pass: the slots of the last PT step hold K only at H != 0
     h->launch_energy_all(2);
-    h->energy_valid = true;
+    h->energy_valid = h->energy_k_valid = true;
     std::vector<int32_t> e(h->n_lat() * 128);
     HIPCHK(hipStreamSynchronize(h->st));
     HIPCHK(hipMemcpy(e.data(), h->d_energy, sizeof(int32_t) * e.size(), hipMemcpyDeviceToHost));
