@@ -215,6 +215,12 @@ sg_status sg_job_times(sg_batch *b, int64_t id, double *t_submit, double *t_admi
 /* spin-flip attempts launched, steps run, arena bytes in use and capacity */
 sg_status sg_batch_stats(sg_batch *b, double *flips, int64_t *steps, int64_t *arena_in_use,
                          int64_t *arena_capacity);
+/* batched form: class PT intervals launched directly and through replays of
+ * captured hipGraphs, and the graphs instantiated.  A class captures its
+ * intervals once its task tables have served 8 intervals unchanged; the
+ * graphs of a class are dropped when a job joins or leaves it. */
+sg_status sg_batch_launch_stats(sg_batch *b, int64_t *intervals_direct,
+                                int64_t *intervals_graphed, int64_t *graphs_built);
 
 /* Serving loop: jobs[i] arrives arrival_s[i] seconds after the call starts
  * (non-decreasing); each is submitted when the host clock reaches its

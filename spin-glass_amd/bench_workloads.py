@@ -78,7 +78,7 @@ def mixed(args):
                        f"seed 0x5EED0003), 32 temperatures geometric in T=[0.2,2.0], 2 replicas, "
                        f"PT every {PT_EVERY}, measurement every 1024, {sweeps} sweeps per group",
            "form": "batched engine: one launch per (L, field) class and PT interval, classes on "
-                   "separate streams",
+                   "separate streams, intervals of a stable class replayed from hipGraphs",
            "plain_flips_per_ns": None if plain is None else round(plain, 3),
            "plain_workload": "same jobs, one launch per job and interval (sweeps / 10)",
            "batched_over_plain": None if plain is None else round(value / plain, 3),

@@ -84,8 +84,40 @@ struct Class {
   DevList<TaskDesc> ptc[2];     // streamed: (pair, chunk) tasks of the PT launch
   hipEvent_t up_ev = nullptr;   // the last upload from the staging
   bool uploaded = false;
+  // Batched intervals captured as hipGraphs: one per (first parity list,
+  // intervals, energy mode of the last interval), valid while the tables
+  // above are.  Graphs of earlier tables wait in `old` until the event
+  // recorded after their last launch has completed.
+  struct Graph { int k, count, emode; hipGraphExec_t exec; };
+  std::vector<Graph> graphs;
+  std::vector<std::pair<hipEvent_t, std::vector<Graph>>> old;
+
+  void retire_graphs() {
+    if (graphs.empty()) return;
+    hipEvent_t e;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(e, st));
+    old.emplace_back(e, std::move(graphs));
+    graphs.clear();
+  }
+  // frees the retired graphs whose launches have finished (all, with wait)
+  void reap_graphs(bool wait) {
+    for (size_t i = 0; i < old.size();) {
+      if (!wait && hipEventQuery(old[i].first) == hipErrorNotReady) {
+        ++i;
+        continue;
+      }
+      hipEventSynchronize(old[i].first);
+      hipEventDestroy(old[i].first);
+      for (auto &g : old[i].second) hipGraphExecDestroy(g.exec);
+      old.erase(old.begin() + (ptrdiff_t)i);
+    }
+  }
 
   void free_all() {
+    if (st) hipStreamSynchronize(st);
+    retire_graphs();
+    reap_graphs(true);
     jt.free_all();
     chunks.free_all();
     slabs.free_all();
@@ -111,6 +143,7 @@ struct sg_batch {
   std::deque<std::vector<hipEvent_t>> inflight;  // per step: one event per busy class
   std::vector<hipEvent_t> ev_pool;
   int64_t next_id = 0, steps = 0;
+  int64_t iv_direct = 0, iv_graphed = 0, graphs_built = 0;   // batched class intervals
   double flips = 0;
 
   ~sg_batch() {
@@ -329,6 +362,8 @@ struct sg_batch {
   static const std::vector<TaskDesc> &pick_ptc(sg_handle &h, int par) { return h.h_ptc[par]; }
 
   void build(Class &c) {
+    c.retire_graphs();   // they hold the old tables' pointers and counts
+    c.reap_graphs(false);
     if (c.uploaded) HIPCHK(hipEventSynchronize(c.up_ev));  // staging free again
     if (c.kernel == SG_KERNEL_STREAMED) {
       c.streamed = true;
@@ -401,58 +436,108 @@ struct sg_batch {
     }
   }
 
-  // one PT interval of every job of the class: PT, pt_every sweeps, the
-  // energies of the final states, the sweep counters
+  // the launches of one batched PT interval of the class: PT, pt_every
+  // sweeps, the energies of the final states (emode as in launch_block), the
+  // sweep counters; k selects the parity list
+  void launch_class_interval(Class &c, int k, int emode) {
+    const int pe = bp.pt_every;
+    if (c.streamed) {
+      const int pm = kPhEnergy | (emode > 1 ? kPhEnergyK : 0);
+      HIPCHK(launch_pt_chunks(c.jt.d, c.ptc[k].d, c.ptc[k].n, c.st));
+      // pairs of sweeps by the slab kernel, a last odd sweep by the colour
+      // launches
+      const int ns = slab_kernel_supported(c.L) ? pe & ~1 : 0;
+      for (int it = 0; it < ns; ++it) {
+        int mode = kPhUpdate;
+        if (it == 0 && ns == pe) mode |= kPhZero;
+        if (it == pe - 1) mode |= pm;
+        HIPCHK(launch_slab(c.L, c.field, c.jt.d, c.slabs.d, c.slabs.n, it & 1, it, mode, c.st));
+      }
+      for (int it = ns; it < pe; ++it)
+        for (int col = 0; col < 2; ++col) {
+          int mode = kPhUpdate;
+          if (it == ns && col == 0) mode |= kPhZero;
+          if (it == pe - 1 && col == 1) mode |= pm;
+          HIPCHK(launch_phase(c.L, c.field, c.jt.d, c.chunks.d, c.chunks.n, col, it, mode, c.st));
+        }
+    } else if (single_kernel_supported(c.L)) {
+      HIPCHK(launch_pt_chunks(c.jt.d, c.ptc[k].d, c.ptc[k].n, c.st));
+      HIPCHK(launch_single(c.L, c.field, c.jt.d, c.tasks[k].d, c.tasks[k].n, pe, emode, c.st));
+    } else {
+      HIPCHK(launch_block(c.L, c.field, c.jt.d, c.tasks[k].d, c.tasks[k].n, 1, pe, emode, c.st));
+    }
+    HIPCHK(launch_advance(c.jt.d, c.jt.n, pe, c.st));
+  }
+
+  // Batched PT intervals of the class that may run back to back from here,
+  // at most `room`: every one but the last ends with no job measuring or
+  // finishing (so the host bookkeeping of the earlier ones launches
+  // nothing).  1, 2 or 4, to keep the number of captured shapes small.
+  int run_length(const Class &c, int room) const {
+    const int64_t pe = bp.pt_every;
+    int64_t n = room;
+    for (const Job *jp : c.jobs) {
+      const sg_handle &h = *jp->h;
+      n = std::min<int64_t>(n, (jp->n_sweeps - h.sweep) / pe);
+      if (h.meas_every > 0) n = std::min<int64_t>(n, (h.meas_every - h.sweep % h.meas_every) / pe);
+    }
+    return n >= 4 ? 4 : n >= 2 ? 2 : 1;
+  }
+
+  // `count` batched intervals from parity list k, the last with energy mode
+  // e_last and the others with e_mid.  Once the class's tables have served
+  // kGraphAfter intervals unchanged, each shape is captured as a hipGraph
+  // and replayed; until then (jobs coming and going) the launches go out
+  // directly, which costs no instantiation.
+  static constexpr int kGraphAfter = 8;
+  void run_batched(Class &c, int k, int count, int e_mid, int e_last) {
+    if (c.steps_since_build < kGraphAfter) {
+      for (int i = 0; i < count; ++i) launch_class_interval(c, k ^ (i & 1), i + 1 < count ? e_mid : e_last);
+      iv_direct += count;
+      return;
+    }
+    hipGraphExec_t exec = nullptr;
+    for (const auto &g : c.graphs)
+      if (g.k == k && g.count == count && g.emode == e_last) exec = g.exec;
+    if (!exec) {
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < count; ++i) launch_class_interval(c, k ^ (i & 1), i + 1 < count ? e_mid : e_last);
+      HIPCHK(hipStreamEndCapture(c.st, &g));
+      const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      HIPCHK(e);
+      c.graphs.push_back(Class::Graph{k, count, e_last, exec});
+      ++graphs_built;
+    }
+    HIPCHK(hipGraphLaunch(exec, c.st));
+    iv_graphed += count;
+  }
+
+  // slice_pt_steps PT intervals of every job of the class
   void run_class(Class &c) {
     const int pe = bp.pt_every;
-    for (int i = 0; i < bp.slice_pt_steps && !c.jobs.empty(); ++i) {
+    for (int i = 0; i < bp.slice_pt_steps && !c.jobs.empty();) {
+      if (bp.form == SG_FORM_BATCHED && c.dirty) build(c);
+      const int count = bp.form == SG_FORM_BATCHED ? run_length(c, bp.slice_pt_steps - i) : 1;
       // U (and K at H != 0) for the next PT step; U and K when a job of the
-      // class measures after this interval
+      // class measures after the last interval
       bool meas = false;
       for (Job *jp : c.jobs)
-        meas = meas || (jp->h->meas_every > 0 && (jp->h->sweep + pe) % jp->h->meas_every == 0);
-      const int emode = c.field || meas ? 2 : 1;
+        meas = meas || (jp->h->meas_every > 0 &&
+                        (jp->h->sweep + (int64_t)count * pe) % jp->h->meas_every == 0);
+      const int e_mid = c.field ? 2 : 1, e_last = c.field || meas ? 2 : 1;
       if (bp.form == SG_FORM_BATCHED) {
-        if (c.dirty) build(c);
-        const int k = (int)(c.steps_since_build & 1);
-        if (c.streamed) {
-          const int pm = kPhEnergy | (emode > 1 ? kPhEnergyK : 0);
-          HIPCHK(launch_pt_chunks(c.jt.d, c.ptc[k].d, c.ptc[k].n, c.st));
-          // pairs of sweeps by the slab kernel, a last odd sweep by the
-          // colour launches
-          const int ns = slab_kernel_supported(c.L) ? pe & ~1 : 0;
-          for (int it = 0; it < ns; ++it) {
-            int mode = kPhUpdate;
-            if (it == 0 && ns == pe) mode |= kPhZero;
-            if (it == pe - 1) mode |= pm;
-            HIPCHK(launch_slab(c.L, c.field, c.jt.d, c.slabs.d, c.slabs.n, it & 1, it, mode,
-                               c.st));
-          }
-          for (int it = ns; it < pe; ++it)
-            for (int col = 0; col < 2; ++col) {
-              int mode = kPhUpdate;
-              if (it == ns && col == 0) mode |= kPhZero;
-              if (it == pe - 1 && col == 1) mode |= pm;
-              HIPCHK(launch_phase(c.L, c.field, c.jt.d, c.chunks.d, c.chunks.n, col, it, mode,
-                                  c.st));
-            }
-        } else if (single_kernel_supported(c.L)) {
-          HIPCHK(launch_pt_chunks(c.jt.d, c.ptc[k].d, c.ptc[k].n, c.st));
-          HIPCHK(launch_single(c.L, c.field, c.jt.d, c.tasks[k].d, c.tasks[k].n, pe, emode,
-                               c.st));
-        } else {
-          HIPCHK(launch_block(c.L, c.field, c.jt.d, c.tasks[k].d, c.tasks[k].n, 1, pe, emode,
-                              c.st));
-        }
-        HIPCHK(launch_advance(c.jt.d, c.jt.n, pe, c.st));
-        ++c.steps_since_build;
+        run_batched(c, (int)(c.steps_since_build & 1), count, e_mid, e_last);
+        c.steps_since_build += count;
       } else {
         for (Job *jp : c.jobs) {
           sg_handle &h = *jp->h;
-          h.launch_interval((int)((h.sweep / pe) & 1), 1, pe, emode);
+          h.launch_interval((int)((h.sweep / pe) & 1), 1, pe, e_last);
         }
       }
-      after_interval(c, emode);
+      for (int r = 0; r < count; ++r) after_interval(c, r + 1 < count ? e_mid : e_last);
+      i += count;
     }
   }
 
@@ -637,6 +722,15 @@ sg_status sg_batch_stats(sg_batch *b, double *flips, int64_t *steps, int64_t *ar
     if (steps) *steps = b->steps;
     if (arena_in_use) *arena_in_use = (int64_t)b->arena.in_use();
     if (arena_capacity) *arena_capacity = (int64_t)b->arena.capacity();
+  });
+}
+
+sg_status sg_batch_launch_stats(sg_batch *b, int64_t *intervals_direct,
+                                int64_t *intervals_graphed, int64_t *graphs_built) {
+  return bguarded(b, [&] {
+    if (intervals_direct) *intervals_direct = b->iv_direct;
+    if (intervals_graphed) *intervals_graphed = b->iv_graphed;
+    if (graphs_built) *graphs_built = b->graphs_built;
   });
 }
 

@@ -57,7 +57,7 @@ EXPORTS = [
     "sg_get_timing", "sg_set_profiling", "sg_get_kernel_stats", "sg_tune_kernel",
     "sg_batch_params_default", "sg_batch_create", "sg_batch_destroy", "sg_batch_last_error",
     "sg_job_footprint", "sg_submit", "sg_step", "sg_drain", "sg_poll", "sg_result",
-    "sg_job_times", "sg_batch_stats", "sg_serve",
+    "sg_job_times", "sg_batch_stats", "sg_batch_launch_stats", "sg_serve",
 ]
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
@@ -125,6 +125,7 @@ def load_library(path: str = LIB_PATH):
     L.sg_result.argtypes = [_vp, ctypes.c_int64, _dp, _u64p]
     L.sg_job_times.argtypes = [_vp, ctypes.c_int64, _dp, _dp, _dp]
     L.sg_batch_stats.argtypes = [_vp, _dp, _i64p, _i64p, _i64p]
+    L.sg_batch_launch_stats.argtypes = [_vp, _i64p, _i64p, _i64p]
     L.sg_serve.argtypes = [_vp, ctypes.POINTER(_Params), _i64p, _dp, ctypes.c_int32, _i64p, _dp]
     _lib = L
     return L
@@ -410,6 +411,15 @@ class Batch:
                                               ctypes.byref(used), ctypes.byref(cap)))
         return {"flips": fl.value, "steps": st.value, "arena_in_use": used.value,
                 "arena_capacity": cap.value}
+
+    def launch_stats(self):
+        """Batched class intervals launched directly and through graph
+        replays, and the graphs instantiated."""
+        d, g, n = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._c(load_library().sg_batch_launch_stats(self._b, ctypes.byref(d), ctypes.byref(g),
+                                                     ctypes.byref(n)))
+        return {"intervals_direct": d.value, "intervals_graphed": g.value,
+                "graphs_built": n.value}
 
     def serve(self, jobs, arrivals):
         """Runs the native serving loop over jobs arriving at `arrivals`

@@ -74,6 +74,53 @@ def test_mix_matches_each_job_alone(alone, form, slice_steps):
     b.close()
 
 
+def _long_mix():
+    """Jobs long enough for the engine to capture and replay its intervals
+    (a class's tables must serve several intervals unchanged first): runs of
+    4, 2 and 1 intervals between measurements, two jobs of one class that
+    finish at different sweeps (the class is rebuilt mid-run), resident and
+    streamed kernels, H != 0."""
+    b4 = geometric_betas(0.3, 2.0, 4)
+    return [
+        sg.Job(L=8, n_sweeps=160, n_samples=64, betas=b4, coupling_seed=101, spin_seed=102,
+               meas_every=12),
+        sg.Job(L=8, n_sweeps=120, n_samples=96, betas=geometric_betas(0.4, 1.5, 3), H=0.25,
+               coupling_seed=111, spin_seed=112, meas_every=40),
+        sg.Job(L=12, n_sweeps=96, n_samples=64, betas=b4, coupling_seed=121, spin_seed=122,
+               meas_every=16),
+        sg.Job(L=12, n_sweeps=200, n_samples=64, betas=geometric_betas(0.2, 2.0, 5),
+               coupling_seed=131, spin_seed=132, meas_every=24),
+        sg.Job(L=20, n_sweeps=96, n_samples=64, betas=b4, coupling_seed=141, spin_seed=142,
+               meas_every=32),
+        sg.Job(L=24, n_sweeps=80, n_samples=64, betas=geometric_betas(0.3, 2.0, 3),
+               coupling_seed=151, spin_seed=152, meas_every=16),
+        sg.Job(L=32, n_sweeps=64, n_samples=64, betas=geometric_betas(0.3, 2.0, 2),
+               coupling_seed=161, spin_seed=162, meas_every=32, kernel=sg.SG_KERNEL_STREAMED),
+    ]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("slice_steps", [4, 3])
+def test_captured_intervals_match_each_job_alone(slice_steps):
+    jobs = _long_mix()
+    ref = [_alone(j) for j in jobs]
+    b = sg.Batch(pt_every=PT, slice_pt_steps=slice_steps, arena_bytes=512 << 20)
+    ids = [b.submit(j) for j in jobs]
+    b.drain()
+    _check_same([b.result(i) for i in ids], ref)
+    assert b.stats()["arena_in_use"] == 0
+    ls = b.launch_stats()
+    # every class interval ran (a class runs while any of its jobs does),
+    # most of them through replays
+    classes = {}
+    for j in jobs:
+        key = (j.L, j.H != 0.0, j.kernel)
+        classes[key] = max(classes.get(key, 0), j.n_sweeps // PT)
+    assert ls["intervals_direct"] + ls["intervals_graphed"] == sum(classes.values())
+    assert ls["intervals_graphed"] > ls["intervals_direct"] and ls["graphs_built"] > 0
+    b.close()
+
+
 @pytest.mark.gpu
 def test_small_arena_queues_jobs_and_keeps_results(alone):
     jobs = _mix()
