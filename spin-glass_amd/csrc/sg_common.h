@@ -147,7 +147,6 @@ struct JobPtrs {
   uint64_t *spins;
   const uint64_t *J;
   const uint64_t *Js;  // coupling site records [group][colour][h][6]
-  uint64_t *rng;       // streamed sweeps: Philox words 2, 3 of each quad, [lattice][N/4]
   const LatDesc *descs;
   const PairDesc *pairs;
   const ThrRec *thr;

@@ -40,12 +40,15 @@ constexpr uint32_t kNoPair = 0xFFFFFFFFu;
 // tasks so that every chunk of a lattice lands on one XCD, the same one in
 // every launch: a lattice written by one colour launch is read by the next
 // from that XCD's L2.  Lattice k of the list (in order of appearance) takes
-// slot k % 8; the slots interleave, short ones padded with no-op tasks
+// slot k % 8; within a slot the tasks go chunk by chunk across its lattices
+// (chunk 0 of each, then chunk 1, ...), so the lattices of one coupling
+// group that share an XCD read each chunk's coupling records close together
+// in time, from L2; the slots interleave, short ones padded with no-op tasks
 // (chunk index past the last quad: no spins, no energy slot touched).
 constexpr int kXcds = 8;
 
 inline std::vector<TaskDesc> xcd_order(const std::vector<TaskDesc> &in, uint32_t pad_chunk) {
-  std::vector<std::vector<TaskDesc>> slot(kXcds);
+  std::vector<std::vector<std::vector<TaskDesc>>> runs(kXcds);   // [slot][lattice][chunk]
   int k = -1;
   uint32_t last_lat = 0xFFFFFFFFu, last_job = 0xFFFFFFFFu;
   for (const TaskDesc &t : in) {
@@ -53,8 +56,17 @@ inline std::vector<TaskDesc> xcd_order(const std::vector<TaskDesc> &in, uint32_t
       ++k;
       last_lat = t.lat[0];
       last_job = t.job;
+      runs[k % kXcds].emplace_back();
     }
-    slot[k % kXcds].push_back(t);
+    runs[k % kXcds].back().push_back(t);
+  }
+  std::vector<std::vector<TaskDesc>> slot(kXcds);
+  for (int x = 0; x < kXcds; ++x) {
+    size_t len = 0;
+    for (const auto &r : runs[x]) len = std::max(len, r.size());
+    for (size_t i = 0; i < len; ++i)
+      for (const auto &r : runs[x])
+        if (i < r.size()) slot[x].push_back(r[i]);
   }
   size_t m = 0;
   for (auto &v : slot) m = std::max(m, v.size());
@@ -145,7 +157,7 @@ inline size_t job_footprint(int L, int n_samples, int R, int T) {
              r(sizeof(ObsAcc) * G * T * 64) + r(16 * (size_t)L) + r(sizeof(JobPtrs));
   if (streamed)
     // chunk tasks: XCD padding adds at most 7 lattices' worth per group
-    b += r(8 * nlat * N / 4) + r(sizeof(TaskDesc) * (nlat + 7 * G) * phase_chunks(L)) +
+    b += r(sizeof(TaskDesc) * (nlat + 7 * G) * phase_chunks(L)) +
          2 * r(sizeof(TaskDesc) * (npairs / 2 + G * R) * pt_chunks(L));
   if (streamed && slab_kernel_supported(L))
     // slab sweeps: the second spin buffer and the (lattice, slab pair) tasks
@@ -213,7 +225,6 @@ struct sg_handle {
   // between PT intervals.
   bool can_stream = false, streamed = false;
   uint64_t *d_Js = nullptr;  // coupling site records [group][colour][h][6]
-  uint64_t *d_rng = nullptr; // streamed: stashed Philox words [lattice][N/4]
   TaskDesc *d_chunks = nullptr, *d_ptc[2] = {nullptr, nullptr};
   std::vector<TaskDesc> h_chunks, h_ptc[2];
   std::vector<int> chunk_off, ptc_off[2];
@@ -313,7 +324,7 @@ struct sg_handle {
     dev_free(d_spins); dev_free(d_J); dev_free(d_thr); dev_free(d_energy); dev_free(d_rec);
     dev_free(d_lat); dev_free(d_pairs); dev_free(d_meas); dev_free(d_jobs); dev_free(d_cnt);
     dev_free(d_acc); dev_free(d_cs); dev_free(d_tasks[0]); dev_free(d_tasks[1]);
-    dev_free(d_self); dev_free(d_Js); dev_free(d_rng); dev_free(d_chunks); dev_free(d_ptc[0]); dev_free(d_ptc[1]);
+    dev_free(d_self); dev_free(d_Js); dev_free(d_chunks); dev_free(d_ptc[0]); dev_free(d_ptc[1]);
     dev_free(d_spins2); dev_free(d_slabs);
     for (auto &g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();
@@ -520,13 +531,12 @@ struct sg_handle {
 
     if (can_stream) {
       build_stream_tasks();
-      dev_alloc(&d_rng, n_lat() * (size_t)(N / 4));
       if (slab_kernel_supported(L)) {
         build_slab_tasks();
         dev_alloc(&d_spins2, n_lat() * N);
       }
     }
-    self.spins = d_spins; self.J = d_J; self.Js = d_Js; self.rng = d_rng;
+    self.spins = d_spins; self.J = d_J; self.Js = d_Js;
     self.spins2 = d_spins2;
     self.descs = d_lat; self.pairs = d_pairs;
     self.thr = d_thr; self.js = d_jobs; self.energy = d_energy; self.counters = d_cnt;

@@ -1689,26 +1689,19 @@ __global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ 
       if (mode & kPhEnergyK) bq[q] = opp[h];
     }
   }
-  // the quad's Philox block: colour 0 computes it, uses words 0 and 1 and
-  // stashes words 2 and 3 for the colour-1 launch of the same sweep
-  SG_GLOBAL uint64_t *__restrict__ stash = to_global(jp.rng) + d.spin_off / 4;
+  // the quad's Philox block, drawn by both colour launches of the sweep:
+  // words 2c and 2c + 1 (recomputing it costs less than passing two words
+  // per quad through memory from one launch to the next)
   uint32_t rw[2] = {0, 0};
   ThrRec tr{};
   if (upd) {
     tr = ld_uniform(jp.thr + d.thr_idx);
     if (active) {
-      if (c == 0) {
-        const uint32_t n = ld_uniform(jp.js).sweep + (uint32_t)it;
-        const u32x4 r =
-            philox_dev((uint32_t)j, n, d.g, (kTagMC << 28) | (d.rt & 0x0FFFFFFFu), d.key0, d.key1);
-        rw[0] = r.v[0];
-        rw[1] = r.v[1];
-        stash[j] = (uint64_t)r.v[2] | ((uint64_t)r.v[3] << 32);
-      } else {
-        const uint64_t w = stash[j];
-        rw[0] = (uint32_t)w;
-        rw[1] = (uint32_t)(w >> 32);
-      }
+      const uint32_t n = ld_uniform(jp.js).sweep + (uint32_t)it;
+      const u32x4 r =
+          philox_dev((uint32_t)j, n, d.g, (kTagMC << 28) | (d.rt & 0x0FFFFFFFu), d.key0, d.key1);
+      rw[0] = c ? r.v[2] : r.v[0];
+      rw[1] = c ? r.v[3] : r.v[1];
     }
   }
   uint64_t AU[1][4] = {{0, 0, 0, 0}}, AK[1][3] = {{0, 0, 0}};

@@ -200,8 +200,7 @@ def test_job_footprint_counts_every_block():
     spins = 8 * G * R * T * N
     J = 8 * G * 3 * N
     site_records = 8 * G * 6 * N          # couplings per site, checkerboard order
-    stash = 8 * G * R * T * (N // 4)      # streamed sweeps: Philox words of each quad
-    major = spins + J + site_records + stash
+    major = spins + J + site_records
     assert major < j.footprint() < 1.1 * major
 
 
