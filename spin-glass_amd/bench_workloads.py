@@ -17,6 +17,10 @@ PT_EVERY = 16
 VALU_PEAK_TOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 W_ALG = 1.22
+# HBM-side bytes per launch of the large-lattice kernels (FETCH_SIZE + WRITE_SIZE, scaled by a
+# copy of known size): k_slab<32> per sweep launch, k_phase<48> per colour launch, 64 lattices;
+# profiles/k_slab_L32_k_phase_L48_64lattices.txt
+TRAFFIC_LARGE = {32: 37.96e6, 48: 109.47e6}
 L_CHOICES = (8, 12, 16, 20, 24)
 
 
@@ -122,6 +126,7 @@ def large(args):
            "per_L": per_l, "form": "batched", "parallelism": "1 GPU"}
     r = _valu_roofline(value, "k_slab<32,false>, k_phase<48,false> (streamed)")
     r["hbm"] = {L: v["hbm_frac"] for L, v in per_l.items()}
+    r["traffic"] = {L: TRAFFIC_LARGE[L] for L in per_l}
     return _base("spin flips/ns, L=32 and 48", value, "spin flips/ns", tot_s, cfg, r,
                  steps=2 * sweeps // PT_EVERY)
 
