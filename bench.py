@@ -64,6 +64,10 @@ def make(samples, form):
     return s
 
 
+SHAPE_NAMES = {sg.SG_KERNEL_RESIDENT: "lattice pairs", sg.SG_KERNEL_RESIDENT_SINGLE: "single lattices",
+               sg.SG_KERNEL_STREAMED: "streamed chunks"}
+
+
 def time_form(samples, form, steps, warmup, profile=False):
     """Times `steps` PT intervals.  profile=False: the whole job as it runs
     (batched: captured intervals replayed).  profile=True: every sweep-kernel
@@ -148,6 +152,9 @@ def main():
 
     b = time_form(SAMPLES_BATCHED, sg.SG_FORM_BATCHED, args.steps, args.warmup)
     value = b["flips"] / b["wall_s"] / 1e9
+    # the work-group shape the autotuner picked for each form (cached per lattice count)
+    shape_b = sg.tune_shapes(L, (SAMPLES_BATCHED // 64) * R * T)
+    shape_p = sg.tune_shapes(L, (SAMPLES_PLAIN // 64) * R * T)
     k = time_form(SAMPLES_BATCHED, sg.SG_FORM_BATCHED, args.steps, args.warmup, profile=True)
     plain = None
     if not args.no_plain:
@@ -178,6 +185,11 @@ def main():
             "plain_flips_per_ns": None if plain is None else round(plain, 3),
             "plain_workload": f"same, {SAMPLES_PLAIN} samples (one group), per-group launches",
             "batched_over_plain": None if plain is None else round(value / plain, 3),
+            "shape": {"batched": SHAPE_NAMES[shape_b[0]], "plain": SHAPE_NAMES[shape_p[0]],
+                      "tuner_ms_batched": {SHAPE_NAMES[m]: round(v, 4)
+                                           for m, v in shape_b[1].items() if v > 0},
+                      "tuner_ms_plain": {SHAPE_NAMES[m]: round(v, 4)
+                                         for m, v in shape_p[1].items() if v > 0}},
             "parallelism": "1 GPU",
         },
         "roofline": {

@@ -72,7 +72,10 @@ enum {
 enum {
   SG_KERNEL_AUTO = 0,     /* chosen per L (and lattice count) from measured rates */
   SG_KERNEL_RESIDENT = 1, /* lattice pair resident in one work-group for a PT interval */
-  SG_KERNEL_STREAMED = 2  /* one launch per colour, lattices streamed in chunks (L >= 12) */
+  SG_KERNEL_STREAMED = 2, /* one launch per colour, lattices streamed in chunks (L >= 12) */
+  SG_KERNEL_RESIDENT_SINGLE = 3 /* one lattice per work-group (twice the work-groups of
+                                   RESIDENT), PT in a launch of its own; the same as
+                                   RESIDENT at L = 22..26, whose kernel holds one lattice */
 };
 
 /* Per-measurement callback: rec holds n_groups*n_temps*64 records of nfield
@@ -152,13 +155,20 @@ sg_status sg_set_profiling(sg_handle *h, int32_t on);
 sg_status sg_get_kernel_stats(sg_handle *h, int32_t kind, int64_t *launches,
                               double *total_ms, double *flips);
 /* the autotuner behind SG_KERNEL_AUTO (no counterpart in the reference): the
- * kernel it picks for n_lattices lattices of size L on a device, with the
- * measured device time of both candidates over the same 64 sweeps (0 when
- * only the resident kernel exists for this L).  The first call for a
- * (device, L, field, power-of-two lattice count) measures; later ones are
- * answered from the process-wide cache.  Errors: sg_last_error(NULL). */
+ * work-group shape (SG_KERNEL_RESIDENT, _RESIDENT_SINGLE or _STREAMED) it
+ * picks for n_lattices lattices of size L on a device, with the measured
+ * device time of the lattice-pair and the streamed shapes over the same 64
+ * sweeps (0 where the L has no such shape).  The first call for a (device,
+ * L, field, power-of-two lattice count) measures every shape the L has;
+ * later ones are answered from the process-wide cache.  Errors:
+ * sg_last_error(NULL). */
 sg_status sg_tune_kernel(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
                          int32_t *kernel, double *resident_ms, double *streamed_ms);
+/* the same decision with the time of every shape: ms[k] for k =
+ * SG_KERNEL_RESIDENT, _STREAMED, _RESIDENT_SINGLE (ms[0] unused; 0 where the
+ * L has no such shape) */
+sg_status sg_tune_shapes(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
+                         int32_t *kernel, double ms[4]);
 
 /* --- batch and serving engine -------------------------------------------
  * No counterpart in the reference, which runs one fixed-size simulation per

@@ -67,18 +67,20 @@ struct DevList {
 // table and the task lists of the class's batched launches.  Every list
 // exists twice: [0] every job at its current PT parity, [1] every parity
 // flipped (the lists alternate from one PT interval to the next).  With
-// SG_KERNEL_AUTO the class picks resident or streamed sweeps whenever its
-// membership changes, from the autotuner's measurement for its lattice count.
+// SG_KERNEL_AUTO the class picks its work-group shape (lattice pairs, single
+// lattices or streamed chunks) whenever its membership changes, from the
+// autotuner's measurement for its lattice count.
 struct Class {
   int L = 0;
   int kernel = SG_KERNEL_AUTO;
   bool field = false, streamed = false;
+  bool single = false;          // one lattice per work-group with a two-lattice kernel
   hipStream_t st = nullptr;
   std::vector<Job *> jobs;
   bool dirty = true;
   int64_t steps_since_build = 0;
   DevList<JobPtrs> jt;
-  DevList<TaskDesc> tasks[2];   // resident: pair/single tasks
+  DevList<TaskDesc> tasks[2];   // resident: pair (or one-lattice) tasks
   DevList<TaskDesc> chunks;     // streamed: (lattice, chunk) tasks
   DevList<TaskDesc> slabs;      // streamed with a slab kernel: (lattice, slab pair) tasks
   DevList<TaskDesc> ptc[2];     // streamed: (pair, chunk) tasks of the PT launch
@@ -217,7 +219,7 @@ struct sg_batch {
     j->p.pt_every = pe;
     j->p.device = bp.device;
     j->p.form = SG_FORM_BATCHED;
-    if (in.kernel < SG_KERNEL_AUTO || in.kernel > SG_KERNEL_STREAMED)
+    if (in.kernel < SG_KERNEL_AUTO || in.kernel > SG_KERNEL_RESIDENT_SINGLE)
       throw Fail{SG_ERR_INVALID, "unknown kernel"};
     if (in.kernel == SG_KERNEL_STREAMED && !phase_kernel_supported(in.L))
       throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
@@ -357,6 +359,7 @@ struct sg_batch {
   }
 
   static const std::vector<TaskDesc> &pick_tasks(sg_handle &h, int par) { return h.h_tasks[par]; }
+  static const std::vector<TaskDesc> &pick_single(sg_handle &h, int) { return h.h_single; }
   static const std::vector<TaskDesc> &pick_chunks(sg_handle &h, int) { return h.h_chunks; }
   static const std::vector<TaskDesc> &pick_slabs(sg_handle &h, int) { return h.h_slabs; }
   static const std::vector<TaskDesc> &pick_ptc(sg_handle &h, int par) { return h.h_ptc[par]; }
@@ -365,16 +368,15 @@ struct sg_batch {
     c.retire_graphs();   // they hold the old tables' pointers and counts
     c.reap_graphs(false);
     if (c.uploaded) HIPCHK(hipEventSynchronize(c.up_ev));  // staging free again
-    if (c.kernel == SG_KERNEL_STREAMED) {
-      c.streamed = true;
-    } else if (c.kernel == SG_KERNEL_RESIDENT || !phase_kernel_supported(c.L)) {
-      c.streamed = false;
-    } else {
+    int mode = c.kernel;
+    if (mode == SG_KERNEL_AUTO) {
       size_t nlat = 0;
       for (Job *j : c.jobs) nlat += j->h->n_lat();
-      c.streamed = tune_streamed(bp.device, c.L, c.field, nlat);
+      mode = tune_kernel_choice(bp.device, c.L, c.field, nlat);
     }
-    for (Job *j : c.jobs) j->h->set_streamed(c.streamed);
+    c.streamed = mode == SG_KERNEL_STREAMED;
+    c.single = mode == SG_KERNEL_RESIDENT_SINGLE && !single_kernel_supported(c.L);
+    for (Job *j : c.jobs) j->h->set_mode(mode);
     reserve(c, c.jt, c.jobs.size());
     c.jt.n = (int)c.jobs.size();
     for (size_t ji = 0; ji < c.jobs.size(); ++ji) c.jt.h[ji] = c.jobs[ji]->h->self;
@@ -392,9 +394,9 @@ struct sg_batch {
       }
     } else {
       for (int k = 0; k < 2; ++k) {
-        gather(c, c.tasks[k], true, pick_tasks, k);
+        gather(c, c.tasks[k], true, c.single ? pick_single : pick_tasks, k);
         upload(c, c.tasks[k]);
-        if (single_kernel_supported(c.L)) {   // PT in its own launch
+        if (single_kernel_supported(c.L) || c.single) {   // PT in its own launch
           gather(c, c.ptc[k], false, pick_ptc, k);
           upload(c, c.ptc[k]);
         }
@@ -463,6 +465,9 @@ struct sg_batch {
     } else if (single_kernel_supported(c.L)) {
       HIPCHK(launch_pt_chunks(c.jt.d, c.ptc[k].d, c.ptc[k].n, c.st));
       HIPCHK(launch_single(c.L, c.field, c.jt.d, c.tasks[k].d, c.tasks[k].n, pe, emode, c.st));
+    } else if (c.single) {
+      HIPCHK(launch_pt_chunks(c.jt.d, c.ptc[k].d, c.ptc[k].n, c.st));
+      HIPCHK(launch_block(c.L, c.field, c.jt.d, c.tasks[k].d, c.tasks[k].n, 0, pe, emode, c.st));
     } else {
       HIPCHK(launch_block(c.L, c.field, c.jt.d, c.tasks[k].d, c.tasks[k].n, 1, pe, emode, c.st));
     }

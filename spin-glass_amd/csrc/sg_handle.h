@@ -135,11 +135,15 @@ inline uint64_t fnv1a(const void *data, size_t n, uint64_t h = 14695981039346656
   return h;
 }
 
-// Autotuner (sg_runtime.cpp): resident (lattice pair in one work-group for
-// a PT interval) or streamed (one launch per colour over chunks of every
-// lattice) sweeps for n_lattices lattices of size L, decided by timing both
-// on the device; cached per (device, L, field, power-of-two lattice count).
-bool tune_streamed(int device, int L, bool field, size_t n_lattices);
+// Autotuner (sg_runtime.cpp): the work-group shape of the sweeps for
+// n_lattices lattices of size L — SG_KERNEL_RESIDENT (a lattice pair in one
+// work-group for a PT interval), SG_KERNEL_RESIDENT_SINGLE (one lattice per
+// work-group, twice the work-groups, PT in its own launch) or
+// SG_KERNEL_STREAMED (one launch per colour over chunks of every lattice) —
+// decided by timing each shape the L has on the device; cached per (device,
+// L, field, power-of-two lattice count).  ms[k] receives the measured time of
+// shape k (0 where the L has no such shape).
+int tune_kernel_choice(int device, int L, bool field, size_t n_lattices, double *ms = nullptr);
 
 // Device bytes one engine job needs, block by block as sg_handle::create
 // allocates them from an arena (each block rounded to the arena's
@@ -155,10 +159,12 @@ inline size_t job_footprint(int L, int n_samples, int R, int T) {
              r(sizeof(JobState)) + r(8 * (size_t)(T > 1 ? T - 1 : 1)) +
              r(sizeof(PairDesc) * npairs) + 2 * r(sizeof(TaskDesc) * nlat) +
              r(sizeof(ObsAcc) * G * T * 64) + r(16 * (size_t)L) + r(sizeof(JobPtrs));
+  // PT launches of the one-lattice shapes: (pair, chunk) tasks per parity
+  b += 2 * r(sizeof(TaskDesc) * (npairs / 2 + G * R) * pt_chunks(L));
+  if (!single_kernel_supported(L)) b += r(sizeof(TaskDesc) * nlat);   // one-lattice tasks
   if (streamed)
     // chunk tasks: XCD padding adds at most 7 lattices' worth per group
-    b += r(sizeof(TaskDesc) * (nlat + 7 * G) * phase_chunks(L)) +
-         2 * r(sizeof(TaskDesc) * (npairs / 2 + G * R) * pt_chunks(L));
+    b += r(sizeof(TaskDesc) * (nlat + 7 * G) * phase_chunks(L));
   if (streamed && slab_kernel_supported(L))
     // slab sweeps: the second spin buffer and the (lattice, slab pair) tasks
     b += r(8 * nlat * N) + r(sizeof(TaskDesc) * (nlat + 7 * G) * kSlabParts);
@@ -251,6 +257,13 @@ struct sg_handle {
   // group (task_off[p][g] .. task_off[p][g+1])
   TaskDesc *d_tasks[2] = {nullptr, nullptr};
   std::vector<int> task_off[2];
+  // one lattice per work-group with a two-lattice kernel
+  // (SG_KERNEL_RESIDENT_SINGLE): every lattice its own task, the same list
+  // for both parities, PT in its own launch; ordered by group (single_off)
+  bool can_single = false, single_shape = false;
+  TaskDesc *d_single = nullptr;
+  std::vector<TaskDesc> h_single;
+  std::vector<int> single_off;
 
   // records for the sink: the measurement kernel writes d_rec, an async copy
   // moves it to one of kRing pinned buffers, and the sink runs once the copy
@@ -323,7 +336,7 @@ struct sg_handle {
   void release() {
     dev_free(d_spins); dev_free(d_J); dev_free(d_thr); dev_free(d_energy); dev_free(d_rec);
     dev_free(d_lat); dev_free(d_pairs); dev_free(d_meas); dev_free(d_jobs); dev_free(d_cnt);
-    dev_free(d_acc); dev_free(d_cs); dev_free(d_tasks[0]); dev_free(d_tasks[1]);
+    dev_free(d_acc); dev_free(d_cs); dev_free(d_tasks[0]); dev_free(d_tasks[1]); dev_free(d_single);
     dev_free(d_self); dev_free(d_Js); dev_free(d_chunks); dev_free(d_ptc[0]); dev_free(d_ptc[1]);
     dev_free(d_spins2); dev_free(d_slabs);
     for (auto &g : graphs) hipGraphExecDestroy(g.exec);
@@ -381,6 +394,7 @@ struct sg_handle {
     nfield = 2 * R + 1 + 3 * L;
 
     can_stream = phase_kernel_supported(L);
+    can_single = !single_kernel_supported(L);
     if (p.kernel == SG_KERNEL_STREAMED && !can_stream)
       throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
     dev_alloc(&d_spins, n_lat() * N);
@@ -529,6 +543,8 @@ struct sg_handle {
     swap_att.assign((size_t)std::max(T - 1, 1), 0);
     reset_counters();
 
+    build_ptc_tasks();
+    if (can_single) build_single_tasks();
     if (can_stream) {
       build_stream_tasks();
       if (slab_kernel_supported(L)) {
@@ -542,24 +558,49 @@ struct sg_handle {
     self.thr = d_thr; self.js = d_jobs; self.energy = d_energy; self.counters = d_cnt;
     dev_alloc(&d_self, 1);
     HIPCHK(hipMemcpy(d_self, &self, sizeof(JobPtrs), hipMemcpyHostToDevice));
-    if (p.kernel == SG_KERNEL_STREAMED)
-      streamed = true;
-    else if (p.kernel == SG_KERNEL_AUTO && can_stream && !arena)
-      streamed = tune_streamed(device, L, field, n_lat());
+    if (p.kernel == SG_KERNEL_AUTO && !arena)
+      set_mode(tune_kernel_choice(device, L, field, n_lat()));
+    else if (p.kernel != SG_KERNEL_AUTO)
+      set_mode(p.kernel);
   }
 
-  // switches between the kernels (between PT intervals); captured graphs
-  // hold the old launch sequence
-  void set_streamed(bool on) {
-    if (on == streamed) return;
-    if (on && !can_stream) throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
-    streamed = on;
+  // switches between the work-group shapes (between PT intervals): k =
+  // SG_KERNEL_RESIDENT, _RESIDENT_SINGLE (the same as _RESIDENT where the
+  // resident kernel holds one lattice) or _STREAMED; captured graphs hold the
+  // old launch sequence
+  void set_mode(int k) {
+    const bool s = k == SG_KERNEL_STREAMED, one = k == SG_KERNEL_RESIDENT_SINGLE && can_single;
+    if (s && !can_stream) throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
+    if (s == streamed && one == single_shape) return;
+    streamed = s;
+    single_shape = one;
     for (auto &g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();
   }
 
+  void build_single_tasks() {
+    h_single.clear();
+    single_off.assign(G + 1, 0);
+    for (int g = 0; g < G; ++g) {
+      single_off[g] = (int)h_single.size();
+      for (int rho = 0; rho < R; ++rho)
+        for (int t = 0; t < T; ++t)
+          h_single.push_back(TaskDesc{{(uint32_t)lat_index(g, rho, t), 0xFFFFFFFFu}, kNoPair, 0});
+    }
+    single_off[G] = (int)h_single.size();
+    if (form == SG_FORM_BATCHED) {   // each coupling group on one XCD
+      const size_t per_group = (size_t)R * T;
+      xcd_interleave(h_single, [per_group](const TaskDesc &k) {
+        return (uint32_t)(k.lat[0] / per_group);
+      });
+    }
+    dev_alloc(&d_single, h_single.size());
+    HIPCHK(hipMemcpy(d_single, h_single.data(), sizeof(TaskDesc) * h_single.size(),
+                     hipMemcpyHostToDevice));
+  }
+
   void build_stream_tasks() {
-    const int nc = phase_chunks(L), np = pt_chunks(L);
+    const int nc = phase_chunks(L);
     h_chunks.clear();
     chunk_off.assign(G + 1, 0);
     const bool per_group = form == SG_FORM_PLAIN;
@@ -580,6 +621,12 @@ struct sg_handle {
     dev_alloc(&d_chunks, h_chunks.size());
     HIPCHK(hipMemcpy(d_chunks, h_chunks.data(), sizeof(TaskDesc) * h_chunks.size(),
                      hipMemcpyHostToDevice));
+  }
+
+  // (pair, chunk) tasks of the PT launches of parity 0 and 1 (streamed and
+  // one-lattice shapes), ordered by group (ptc_off)
+  void build_ptc_tasks() {
+    const int np = pt_chunks(L);
     for (int par = 0; par < 2; ++par) {
       h_ptc[par].clear();
       ptc_off[par].assign(G + 1, 0);
@@ -726,20 +773,24 @@ struct sg_handle {
     }
     const bool per_group = form == SG_FORM_PLAIN;
     const bool single = single_kernel_supported(L);
+    // one lattice per work-group: PT in its own launch first
+    const bool own_pt = single || single_shape;
+    const TaskDesc *tasks = single_shape ? d_single : d_tasks[par];
+    const std::vector<int> &off = single_shape ? single_off : task_off[par];
     for (int g = 0; g < (per_group ? G : 1); ++g) {
-      const int a = per_group ? task_off[par][g] : 0;
-      const int b = per_group ? task_off[par][g + 1] : task_off[par][G];
+      const int a = per_group ? off[g] : 0;
+      const int b = per_group ? off[g + 1] : off[G];
       const double fl = (double)seg * N * 64.0 * R * T * (per_group ? 1 : G);
-      if (single && do_pt) {
+      if (own_pt && do_pt) {
         const int pa = per_group ? ptc_off[par][g] : 0;
         const int pb = per_group ? ptc_off[par][g + 1] : ptc_off[par][G];
         HIPCHK(launch_pt_chunks(d_self, d_ptc[par] + pa, pb - pa, st));
       }
       timed(0, fl, [&] {
         if (single)
-          HIPCHK(launch_single(L, field, d_self, d_tasks[par] + a, b - a, seg, want_energy, st));
+          HIPCHK(launch_single(L, field, d_self, tasks + a, b - a, seg, want_energy, st));
         else
-          HIPCHK(launch_block(L, field, d_self, d_tasks[par] + a, b - a, do_pt, seg,
+          HIPCHK(launch_block(L, field, d_self, tasks + a, b - a, own_pt ? 0 : do_pt, seg,
                               want_energy, st));
       });
     }

@@ -9,15 +9,17 @@ using namespace sg;
 using namespace sg::detail;
 
 // ---------------------------------------------------------------- autotuner
-// Resident or streamed sweeps for (device, L, field, lattice count rounded up
-// to a power of two): both run the same PT intervals on a scratch handle of
-// that shape (random couplings and spins, 24 temperatures or fewer, 2
-// replicas, PT every 16, the batched form's captured intervals) and the
-// faster one is kept for the life of the process.
+// Work-group shape of the sweeps for (device, L, field, lattice count rounded
+// up to a power of two): lattice pairs, single lattices (where the resident
+// kernel holds two) and streamed chunks (where the L has a streamed kernel)
+// run the same PT intervals on a scratch handle of that shape (random
+// couplings and spins, 24 temperatures or fewer, 2 replicas, PT every 16,
+// the batched form's captured intervals) and the fastest is kept for the
+// life of the process.
 namespace {
 struct TuneRes {
-  bool streamed;
-  double resident_ms, streamed_ms;
+  int kernel;
+  double ms[4];   // by SG_KERNEL_* value; 0 where not measured
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, bool, size_t>, TuneRes> g_tune;
@@ -35,29 +37,38 @@ TuneRes tune_measure(int device, int L, bool field, size_t nlat) {
   sg_handle h;
   h.create(p);
   h.init_lattice();
-  double ms[2];
-  for (int mode = 0; mode < 2; ++mode) {
-    h.set_streamed(mode == 1);
-    h.run(32, nullptr, nullptr);  // warm-up: code, caches, captured graphs
+  std::vector<int> modes = {SG_KERNEL_RESIDENT};
+  if (h.can_single) modes.push_back(SG_KERNEL_RESIDENT_SINGLE);
+  if (h.can_stream) modes.push_back(SG_KERNEL_STREAMED);
+  TuneRes res{SG_KERNEL_RESIDENT, {0, 0, 0, 0}};
+  for (int m : modes) {
+    h.set_mode(m);
+    h.run(64, nullptr, nullptr);  // warm-up: code, caches, the same captured graphs
     const double before = h.kernel_ms;
     h.run(64, nullptr, nullptr);
-    ms[mode] = h.kernel_ms - before;
+    res.ms[m] = h.kernel_ms - before;
+    if (res.ms[m] < res.ms[res.kernel]) res.kernel = m;
   }
-  return TuneRes{ms[1] < ms[0], ms[0], ms[1]};
+  return res;
 }
-}  // namespace
 
-namespace sg {
-namespace detail {
-bool tune_streamed(int device, int L, bool field, size_t n_lattices) {
-  if (!phase_kernel_supported(L)) return false;
+const TuneRes &tune_lookup(int device, int L, bool field, size_t n_lattices) {
   size_t bucket = 1;
   while (bucket < n_lattices) bucket <<= 1;
   std::lock_guard<std::mutex> lock(g_tune_mu);
   const auto key = std::make_tuple(device, L, field, bucket);
   auto it = g_tune.find(key);
   if (it == g_tune.end()) it = g_tune.emplace(key, tune_measure(device, L, field, bucket)).first;
-  return it->second.streamed;
+  return it->second;   // entries are never erased
+}
+}  // namespace
+
+namespace sg {
+namespace detail {
+int tune_kernel_choice(int device, int L, bool field, size_t n_lattices, double *ms) {
+  const TuneRes &r = tune_lookup(device, L, field, n_lattices);
+  if (ms) std::copy(r.ms, r.ms + 4, ms);
+  return r.kernel;
 }
 }  // namespace detail
 }  // namespace sg
@@ -106,7 +117,8 @@ sg_status sg_create(const sg_params *p, sg_handle **out) {
   if (p->pt_every < 0 || p->meas_every < 0) return bad("negative cadence");
   if (!p->betas && !(p->T_min > 0.0 && p->T_max > 0.0)) return bad("need betas or T_min/T_max > 0");
   if (p->form != SG_FORM_PLAIN && p->form != SG_FORM_BATCHED) return bad("unknown form");
-  if (p->kernel < SG_KERNEL_AUTO || p->kernel > SG_KERNEL_STREAMED) return bad("unknown kernel");
+  if (p->kernel < SG_KERNEL_AUTO || p->kernel > SG_KERNEL_RESIDENT_SINGLE)
+    return bad("unknown kernel");
   if (p->kernel == SG_KERNEL_STREAMED && !phase_kernel_supported(p->L))
     return bad("no streamed kernel for this L");
   sg_handle *h = new (std::nothrow) sg_handle();
@@ -241,20 +253,23 @@ sg_status sg_tune_kernel(int32_t device, int32_t L, int32_t n_lattices, int32_t 
   g_create_error.clear();
   if (!kernel || n_lattices < 1 || L < 2 || L > kMaxL || (L & 1)) return SG_ERR_INVALID;
   try {
-    if (!phase_kernel_supported(L)) {
-      *kernel = SG_KERNEL_RESIDENT;
-      if (resident_ms) *resident_ms = 0;
-      if (streamed_ms) *streamed_ms = 0;
-      return SG_OK;
-    }
-    tune_streamed(device, L, field != 0, (size_t)n_lattices);
-    size_t bucket = 1;
-    while (bucket < (size_t)n_lattices) bucket <<= 1;
-    std::lock_guard<std::mutex> lock(g_tune_mu);
-    const TuneRes &r = g_tune.at(std::make_tuple(device, L, field != 0, bucket));
-    *kernel = r.streamed ? SG_KERNEL_STREAMED : SG_KERNEL_RESIDENT;
-    if (resident_ms) *resident_ms = r.resident_ms;
-    if (streamed_ms) *streamed_ms = r.streamed_ms;
+    double ms[4];
+    *kernel = tune_kernel_choice(device, L, field != 0, (size_t)n_lattices, ms);
+    if (resident_ms) *resident_ms = ms[SG_KERNEL_RESIDENT];
+    if (streamed_ms) *streamed_ms = ms[SG_KERNEL_STREAMED];
+    return SG_OK;
+  } catch (const Fail &e) {
+    g_create_error = e.msg;
+    return e.code;
+  }
+}
+
+sg_status sg_tune_shapes(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
+                         int32_t *kernel, double ms[4]) {
+  g_create_error.clear();
+  if (!kernel || !ms || n_lattices < 1 || L < 2 || L > kMaxL || (L & 1)) return SG_ERR_INVALID;
+  try {
+    *kernel = tune_kernel_choice(device, L, field != 0, (size_t)n_lattices, ms);
     return SG_OK;
   } catch (const Fail &e) {
     g_create_error = e.msg;

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libsg.so")
 
 SG_FORM_PLAIN = 0
 SG_FORM_BATCHED = 1
-SG_KERNEL_AUTO, SG_KERNEL_RESIDENT, SG_KERNEL_STREAMED = 0, 1, 2
+SG_KERNEL_AUTO, SG_KERNEL_RESIDENT, SG_KERNEL_STREAMED, SG_KERNEL_RESIDENT_SINGLE = 0, 1, 2, 3
 RES_COLUMNS = ["beta", "e", "m", "q", "q2", "q4", "binder", "chi0", "chi1", "xi_L", "n_meas"]
 NRES = len(RES_COLUMNS)
 
@@ -54,7 +54,7 @@ EXPORTS = [
     "sg_run", "sg_save_checkpoint", "sg_load_checkpoint", "sg_nfield", "sg_n_groups",
     "sg_sweep_count", "sg_get_betas", "sg_get_spins", "sg_set_spins", "sg_get_couplings",
     "sg_set_couplings", "sg_get_energies", "sg_get_swap_counts", "sg_get_results",
-    "sg_get_timing", "sg_set_profiling", "sg_get_kernel_stats", "sg_tune_kernel",
+    "sg_get_timing", "sg_set_profiling", "sg_get_kernel_stats", "sg_tune_kernel", "sg_tune_shapes",
     "sg_batch_params_default", "sg_batch_create", "sg_batch_destroy", "sg_batch_last_error",
     "sg_job_footprint", "sg_submit", "sg_step", "sg_drain", "sg_poll", "sg_result",
     "sg_job_times", "sg_batch_stats", "sg_batch_launch_stats", "sg_serve",
@@ -110,6 +110,8 @@ def load_library(path: str = LIB_PATH):
     L.sg_get_kernel_stats.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), _dp, _dp]
     L.sg_tune_kernel.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                  _i32p, _dp, _dp]
+    L.sg_tune_shapes.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                 _i32p, _dp]
     _i64p = ctypes.POINTER(ctypes.c_int64)
     L.sg_batch_params_default.argtypes = [ctypes.POINTER(_BatchParams)]
     L.sg_batch_create.argtypes = [ctypes.POINTER(_BatchParams), ctypes.POINTER(_vp)]
@@ -292,13 +294,26 @@ class Sim:
 
 def tune_kernel(L, n_lattices, field=False, device=0):
     """The autotuner's choice for n_lattices lattices of size L:
-    (SG_KERNEL_RESIDENT or SG_KERNEL_STREAMED, resident ms, streamed ms)."""
+    (SG_KERNEL_RESIDENT, _STREAMED or _RESIDENT_SINGLE, lattice-pair ms,
+    streamed ms)."""
     lib = load_library()
     k = ctypes.c_int32()
     r, st = ctypes.c_double(), ctypes.c_double()
     _check(lib.sg_tune_kernel(device, L, n_lattices, 1 if field else 0, ctypes.byref(k),
                               ctypes.byref(r), ctypes.byref(st)))
     return k.value, r.value, st.value
+
+
+def tune_shapes(L, n_lattices, field=False, device=0):
+    """The autotuner's choice and the measured ms of every shape:
+    (kernel, {SG_KERNEL_RESIDENT: ms, SG_KERNEL_STREAMED: ms,
+    SG_KERNEL_RESIDENT_SINGLE: ms}), 0 ms where the L has no such shape."""
+    lib = load_library()
+    k = ctypes.c_int32()
+    ms = (ctypes.c_double * 4)()
+    _check(lib.sg_tune_shapes(device, L, n_lattices, 1 if field else 0, ctypes.byref(k), ms))
+    return k.value, {SG_KERNEL_RESIDENT: ms[1], SG_KERNEL_STREAMED: ms[2],
+                     SG_KERNEL_RESIDENT_SINGLE: ms[3]}
 
 
 # ------------------------------------------------------------------ engine

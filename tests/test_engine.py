@@ -78,8 +78,8 @@ def _long_mix():
     """Jobs long enough for the engine to capture and replay its intervals
     (a class's tables must serve several intervals unchanged first): runs of
     4, 2 and 1 intervals between measurements, two jobs of one class that
-    finish at different sweeps (the class is rebuilt mid-run), resident and
-    streamed kernels, H != 0."""
+    finish at different sweeps (the class is rebuilt mid-run), resident,
+    one-lattice and streamed shapes, H != 0."""
     b4 = geometric_betas(0.3, 2.0, 4)
     return [
         sg.Job(L=8, n_sweeps=160, n_samples=64, betas=b4, coupling_seed=101, spin_seed=102,
@@ -96,6 +96,12 @@ def _long_mix():
                coupling_seed=151, spin_seed=152, meas_every=16),
         sg.Job(L=32, n_sweeps=64, n_samples=64, betas=geometric_betas(0.3, 2.0, 2),
                coupling_seed=161, spin_seed=162, meas_every=32, kernel=sg.SG_KERNEL_STREAMED),
+        sg.Job(L=16, n_sweeps=96, n_samples=100, betas=geometric_betas(0.3, 2.0, 5),
+               coupling_seed=171, spin_seed=172, meas_every=24,
+               kernel=sg.SG_KERNEL_RESIDENT_SINGLE),
+        sg.Job(L=20, n_sweeps=64, n_samples=64, betas=geometric_betas(0.3, 2.0, 3), H=0.5,
+               coupling_seed=181, spin_seed=182, meas_every=32,
+               kernel=sg.SG_KERNEL_RESIDENT_SINGLE),
     ]
 
 
@@ -205,10 +211,15 @@ def test_job_footprint_counts_every_block():
 
 
 @pytest.mark.gpu
-def test_autotuner_measures_both_kernels_and_caches():
-    k, r, s = sg.tune_kernel(8, 64)
-    assert k == sg.SG_KERNEL_RESIDENT and r == 0 and s == 0     # no streamed kernel below L=12
-    k, r, s = sg.tune_kernel(32, 64)
-    assert k in (sg.SG_KERNEL_RESIDENT, sg.SG_KERNEL_STREAMED) and r > 0 and s > 0
-    assert (k == sg.SG_KERNEL_STREAMED) == (s < r)
-    assert sg.tune_kernel(32, 60) == (k, r, s)                  # same power-of-two bucket
+def test_autotuner_measures_every_shape_and_caches():
+    P, S, O = sg.SG_KERNEL_RESIDENT, sg.SG_KERNEL_STREAMED, sg.SG_KERNEL_RESIDENT_SINGLE
+    k, ms = sg.tune_shapes(8, 64)
+    assert ms[P] > 0 and ms[O] > 0 and ms[S] == 0                # no streamed kernel below L=12
+    assert k == min((P, O), key=lambda m: ms[m])
+    k, ms = sg.tune_shapes(24, 64)                               # k_one: one shape resident
+    assert ms[P] > 0 and ms[O] == 0 and ms[S] > 0 and k in (P, S)
+    k, ms = sg.tune_shapes(32, 64)
+    assert ms[P] > 0 and ms[O] > 0 and ms[S] > 0
+    assert k == min((P, O, S), key=lambda m: ms[m])
+    assert sg.tune_shapes(32, 60) == (k, ms)                     # same power-of-two bucket
+    assert sg.tune_kernel(32, 64) == (k, ms[P], ms[S])

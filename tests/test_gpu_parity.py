@@ -112,6 +112,26 @@ def test_edge_shapes_match_oracle(case):
         np.testing.assert_array_equal(rec, orc.meas(i))
 
 
+@pytest.mark.parametrize("form", [sg.SG_FORM_PLAIN, sg.SG_FORM_BATCHED])
+@pytest.mark.parametrize("L,H", [(2, 0.0), (4, 0.0), (8, 0.0), (12, 0.5), (16, 0.0), (18, 0.0),
+                                 (20, 0.25), (28, 0.0), (32, 0.0)])
+def test_single_lattice_shape_matches_oracle(L, H, form):
+    """One lattice per work-group with the two-lattice kernels (k_block,
+    k_mid, k_big): every lattice its own task, PT in its own launch; 5
+    temperatures (pairs of both parities leave different slots out), one
+    full and one partial lane group, measurements between PT steps."""
+    betas = geometric_betas(0.3, 2.0, 5)
+    sim, orc = _pair(L, 80, 2, betas, H=H, pt_every=2, meas_every=3, form=form,
+                     kernel=sg.SG_KERNEL_RESIDENT_SINGLE)
+    for n in (4, 3, 5):
+        sim.run(n)
+        orc.run(n)
+        _assert_same(sim, orc)
+    assert len(sim.records) == orc.n_meas() == 4
+    for i, (_, rec) in enumerate(sim.records):
+        np.testing.assert_array_equal(rec, orc.meas(i))
+
+
 @pytest.mark.parametrize("L", [12, 16, 18, 24, 32, 40, 48])
 def test_streamed_kernel_matches_oracle(L):
     """One launch per colour over (lattice, chunk) tasks, PT in its own
