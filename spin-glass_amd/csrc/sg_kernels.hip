@@ -545,9 +545,13 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   // neighbour indices within the opposite colour's half (y, z: same for both
   // colours; x: per colour) and the 6 couplings of each owned site
   // (couplings from the site records: three 16-byte loads per site,
-  // contiguous across the wave)
+  // contiguous across the wave).  One x neighbour of a site is the
+  // opposite-colour site of the same index h, which this thread owns: its
+  // value comes from a register, and its coupling goes first (the bond
+  // count is symmetric in the six (neighbour, coupling) pairs); xo is the
+  // index of the other x neighbour.
   const SG_GLOBAL char *Js = (const SG_GLOBAL char *)(to_global(jp.Js) + 2 * d0.j_off);
-  int yz[2][4], xi[2][2][2];
+  int yz[2][4], xo[2][2];
   uint64_t jw[2][2][6];
   if (active) {
 #pragma unroll
@@ -562,11 +566,18 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
       yz[q][3] = xh + G::LH * (y + L * zm);
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        const int x = 2 * xh + ((y + z + c) & 1);
-        const int xp = (x + 1) % L, xm = (x + L - 1) % L;
-        xi[q][c][0] = (xp >> 1) + G::LH * (y + L * z);
-        xi[q][c][1] = (xm >> 1) + G::LH * (y + L * z);
-        site_record_free(Js, c * G::NH + h, jw[q][c]);
+        // x even: x + 1 is (xh, y, z) of the other colour, x - 1 the other
+        // neighbour; x odd: the reverse
+        const bool odd = ((y + z + c) & 1) != 0;
+        const int x = 2 * xh + (odd ? 1 : 0);
+        const int xn = odd ? (x + 1) % L : (x + L - 1) % L;
+        xo[q][c] = (xn >> 1) + G::LH * (y + L * z);
+        uint64_t r[6];
+        site_record_free(Js, c * G::NH + h, r);
+        jw[q][c][0] = odd ? r[1] : r[0];
+        jw[q][c][1] = odd ? r[0] : r[1];
+#pragma unroll
+        for (int k = 2; k < 6; ++k) jw[q][c][k] = r[k];
       }
     }
   }
@@ -604,6 +615,19 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   }
   __syncthreads();
 
+  // the thread's own sites of both colours, held in registers for the launch
+  uint64_t own[2][2][2] = {};   // [lattice][colour][q]
+  if (active) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      if (a == 1 && !two) break;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        own[a][0][q] = latB[a][tid + q * G::Q];
+        own[a][1][q] = latW[a][tid + q * G::Q];
+      }
+    }
+  }
   const uint32_t sw0 = (kTagMC << 28) | (d0.rt & 0x0FFFFFFFu);
   const uint32_t sw1 = (kTagMC << 28) | (d1.rt & 0x0FFFFFFFu);
   for (int it = 0; it < nsweeps; ++it) {
@@ -624,15 +648,16 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
 #pragma unroll
           for (int q = 0; q < 2; ++q) {
             uint64_t nv[6];
-            nv[0] = opp[xi[q][c][0]];
-            nv[1] = opp[xi[q][c][1]];
+            nv[0] = own[a][1 - c][q];
+            nv[1] = opp[xo[q][c]];
             nv[2] = opp[yz[q][0]];
             nv[3] = opp[yz[q][1]];
             nv[4] = opp[yz[q][2]];
             nv[5] = opp[yz[q][3]];
-            uint64_t s = mine[tid + q * G::Q];
+            uint64_t s = own[a][c][q];
             update_site<L, FIELD>(s, nv, jw[q][c], r.v[2 * c + q], tr);
             mine[tid + q * G::Q] = s;
+            own[a][c][q] = s;
           }
         }
       }
@@ -652,10 +677,10 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
       const uint64_t *opp = latW[a];
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        const uint64_t s = latB[a][tid + q * G::Q];
+        const uint64_t s = own[a][0][q];
         uint64_t nv[6];
-        nv[0] = opp[xi[q][0][0]];
-        nv[1] = opp[xi[q][0][1]];
+        nv[0] = own[a][1][q];
+        nv[1] = opp[xo[q][0]];
         nv[2] = opp[yz[q][0]];
         nv[3] = opp[yz[q][1]];
         nv[4] = opp[yz[q][2]];
@@ -665,7 +690,7 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
         vc_add3(AU[a], u0, u1, u2);
         if (want_energy > 1) {
           vc_add1(AK[a], s);
-          vc_add1(AK[a], latW[a][tid + q * G::Q]);
+          vc_add1(AK[a], own[a][1][q]);
         }
       }
     }
