@@ -52,7 +52,7 @@ HBM_PEAK_GBS = 8000.0
 # 16 sweeps): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of that kernel, each
 # scaled by what the counter reads for a device copy of known size (the
 # --copy-probe run); profiles/k_block_L16_batched_2048_traffic.txt.
-TRAFFIC_L16 = 109.2e6
+TRAFFIC_L16 = 109.0e6
 
 
 def make(samples, form):
