@@ -703,6 +703,8 @@ struct sg_handle {
         nfin = (uint32_t)(u + 1);
       }
       r.nfin[sig] = nfin;
+      // the sweep kernels take u = 5, 6 as always accepted (true for H <= 4)
+      if (nfin > 5) throw Fail{SG_ERR_INVALID, "H out of range [0, 4]"};
     }
     if (!field && (r.nfin[0] != 3 || r.nfin[1] != 3))
       throw Fail{SG_ERR_INVALID, "unexpected threshold table at H = 0"};

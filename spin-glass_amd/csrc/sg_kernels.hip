@@ -48,6 +48,38 @@ __device__ __forceinline__ uint32_t metropolis32(uint32_t s, const uint32_t (&n)
   return SG_BITOP3(s, u2, mid, kXorOr);               // s ^ (u >= 4 | mid)
 }
 
+// The same at H != 0: a lane flips iff rr < T[sigma][u], sigma its spin bit
+// and u its unsatisfied bonds.  R[sigma][u] (u = 0..4) is all-ones iff the
+// site's random number is below T[sigma][u] (or that threshold is 2^32); u =
+// 5, 6 always flip, since dE = 2H sigma' - 8 <= 0 for H <= 4.  25 VALU ops.
+constexpr uint32_t kOr3 = (kA | kB | kC) & 0xFF;                       // a | b | c
+constexpr uint32_t kSelNot = ((kA & ~kB) | (~kA & kC)) & 0xFF;         // a ? ~b : c
+__device__ __forceinline__ uint32_t metropolis32_field(uint32_t s, const uint32_t (&n)[6],
+                                                       const uint32_t (&j)[6],
+                                                       const uint32_t (&R)[2][5]) {
+  const uint32_t b0 = SG_BITOP3(s, n[0], j[0], kXor3);
+  const uint32_t b1 = SG_BITOP3(s, n[1], j[1], kXor3);
+  const uint32_t b2 = SG_BITOP3(s, n[2], j[2], kXor3);
+  const uint32_t b3 = SG_BITOP3(s, n[3], j[3], kXor3);
+  const uint32_t b4 = SG_BITOP3(s, n[4], j[4], kXor3);
+  const uint32_t b5 = SG_BITOP3(s, n[5], j[5], kXor3);
+  const uint32_t s1 = SG_BITOP3(b0, b1, b2, kXor3), c1 = SG_BITOP3(b0, b1, b2, kMaj3);
+  const uint32_t s2 = SG_BITOP3(b3, b4, b5, kXor3), c2 = SG_BITOP3(b3, b4, b5, kMaj3);
+  const uint32_t u0 = s1 ^ s2, c3 = s1 & s2;
+  const uint32_t u1 = SG_BITOP3(c1, c2, c3, kXor3), u2 = SG_BITOP3(c1, c2, c3, kMaj3);
+  uint32_t F[2];
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    // u = 0..3 (u2 = 0): R[g][2 u1 + u0]; u = 4..6 (u2 = 1): u = 4 iff u1 = u0 = 0
+    const uint32_t lo = SG_BITOP3(u0, R[g][1], R[g][0], kSel);
+    const uint32_t hi = SG_BITOP3(u0, R[g][3], R[g][2], kSel);
+    const uint32_t m = SG_BITOP3(u1, hi, lo, kSel);
+    const uint32_t t = SG_BITOP3(u1, u0, R[g][4], kOr3);
+    F[g] = SG_BITOP3(u2, t, m, kSel);
+  }
+  return SG_BITOP3(s, F[1], F[0], kSelNot);   // s = 1: s ^ F1 = ~F1; s = 0: F0
+}
+
 // Philox4x32-10 with the key XORs as single 3-input ops (same numbers as
 // sg::philox).
 __device__ __forceinline__ u32x4 philox_dev(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -396,20 +428,6 @@ __device__ void wg_count_lanes(const uint64_t (&A)[NL][P0], uint64_t *fold, int3
   }
 }
 
-// lanes whose bit-sliced count (u2 u1 u0) is >= k (k in 0..7, per thread)
-__device__ __forceinline__ uint64_t count_ge(uint64_t u2, uint64_t u1, uint64_t u0, int k) {
-  const uint64_t k2 = 0 - (uint64_t)((k >> 2) & 1);
-  const uint64_t k1 = 0 - (uint64_t)((k >> 1) & 1);
-  const uint64_t k0 = 0 - (uint64_t)(k & 1);
-  uint64_t gt = u2 & ~k2;
-  uint64_t eq = ~(u2 ^ k2);
-  gt |= eq & u1 & ~k1;
-  eq &= ~(u1 ^ k1);
-  gt |= eq & u0 & ~k0;
-  eq &= ~(u0 ^ k0);
-  return gt | eq;
-}
-
 // ------------------------------------------------------------- sweep kernel
 // One work-group per task: one lattice, or two lattices of adjacent
 // temperature slots t, t+1 of the same (group, replica).  Both share the
@@ -444,25 +462,24 @@ __device__ __forceinline__ void update_site(uint64_t &s, const uint64_t (&nv)[6]
     const uint32_t hi = metropolis32((uint32_t)(s >> 32), nhi, jhi, R0, R1, R2);
     s = (uint64_t)lo | ((uint64_t)hi << 32);
   } else {
-    const uint64_t b0 = s ^ nv[0] ^ jw[0];
-    const uint64_t b1 = s ^ nv[1] ^ jw[1];
-    const uint64_t b2 = s ^ nv[2] ^ jw[2];
-    const uint64_t b3 = s ^ nv[3] ^ jw[3];
-    const uint64_t b4 = s ^ nv[4] ^ jw[4];
-    const uint64_t b5 = s ^ nv[5] ^ jw[5];
-    const uint64_t s1 = b0 ^ b1 ^ b2, c1 = maj3(b0, b1, b2);
-    const uint64_t s2 = b3 ^ b4 ^ b5, c2 = maj3(b3, b4, b5);
-    const uint64_t u0 = s1 ^ s2, c3 = s1 & s2;
-    const uint64_t u1 = c1 ^ c2 ^ c3, u2 = maj3(c1, c2, c3);
-    int k0 = 0, k1 = 0;
+    // thresholds past nfin are 2^32: always below (the host keeps nfin <= 5)
+    uint32_t R[2][5];
 #pragma unroll
-    for (int u = 0; u < 7; ++u) {
-      k0 += ((uint32_t)u < tr.nfin[0]) & (rr >= tr.t[0][u]);
-      k1 += ((uint32_t)u < tr.nfin[1]) & (rr >= tr.t[1][u]);
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int u = 0; u < 5; ++u)
+        R[g][u] = ((uint32_t)u >= tr.nfin[g] || rr < tr.t[g][u]) ? ~0u : 0u;
+    uint32_t nlo[6], nhi[6], jlo[6], jhi[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      nlo[k] = (uint32_t)nv[k];
+      nhi[k] = (uint32_t)(nv[k] >> 32);
+      jlo[k] = (uint32_t)jw[k];
+      jhi[k] = (uint32_t)(jw[k] >> 32);
     }
-    const uint64_t f0 = count_ge(u2, u1, u0, k0);
-    const uint64_t f1 = count_ge(u2, u1, u0, k1);
-    s ^= (s & f1) | (~s & f0);
+    const uint32_t lo = metropolis32_field((uint32_t)s, nlo, jlo, R);
+    const uint32_t hi = metropolis32_field((uint32_t)(s >> 32), nhi, jhi, R);
+    s = (uint64_t)lo | ((uint64_t)hi << 32);
   }
 }
 

@@ -191,8 +191,12 @@ def test_large_lattice_field_matches_oracle(L):
     np.testing.assert_array_equal(sim.records[0][1], orc.meas(0))
 
 
-@pytest.mark.parametrize("H", [0.0625, 0.5, 2.5])
+@pytest.mark.parametrize("H", [0.0625, 0.5, 2.0, 2.5, 4.0])
 def test_field_matches_oracle(H):
+    """H != 0 thresholds by spin bit and unsatisfied bonds u: at H = 2 the
+    u = 4 move of an up spin costs nothing (four finite thresholds for one
+    spin value, five for the other), at H = 2.5 both have a finite u = 4
+    threshold, at H = 4 (the largest H accepted) u = 5 costs nothing."""
     betas = geometric_betas(0.3, 2.0, 5)
     sim, orc = _pair(8, 64, 2, betas, H=H, pt_every=1, meas_every=4)
     sim.run(9)
