@@ -57,26 +57,66 @@ def run(L, H, samples, sweeps, out_dir, seed0=0x5EED0201, progress=True):
     for done in range(0, sweeps, chunk):
         sim.run(min(chunk, sweeps - done), sink=sink)
     n = count[1]
-    cols = analysis.estimators_from_time_means(acc["q0"] / n, acc["re"] / n, acc["im"] / n,
-                                               acc["e"] / n, acc["m"] / n, L)
+    means = [acc[k] / n for k in ("q0", "re", "im", "e", "m")]
+    cols = analysis.estimators_from_time_means(*means, L)
     sub = os.path.join(out_dir, f"{L:02}")
     os.makedirs(sub, exist_ok=True)
     path = os.path.join(sub, f"{H:06.4f}-res.dat")
     analysis.write_res(path, analysis.ladder(T), cols, 2 * n * 1024)
+    if samples % 64 == 0 and samples >= 128:
+        jk = group_jackknife_errors(means, L)
+        np.savetxt(jk_path(path), np.column_stack([analysis.ladder(T), jk.T]), fmt="%16.12f",
+                   header="beta, sigma of chi0, xi_L/L, e, m: leave-one-lane-group-out jackknife")
     ms, flips = sim.timing()
     return path, flips / (ms * 1e6)
 
 
+def jk_path(res_path):
+    return res_path[:-len("-res.dat")] + "-res.jk.dat"
+
+
+def group_jackknife_errors(means, L, block=64):
+    """Errors of chi0, xi_L/L, e, m ([4][T]) with the 64-sample lane groups
+    as the independent units.  The 64 samples of one uint64 word share the
+    random number of every site update, so their thermal fluctuations are
+    correlated and the table's own formula (variance over samples divided by
+    the sample count, analyze-all.py:44-68) understates the error where
+    thermal noise dominates (high temperature).  Leave-one-group-out
+    jackknife over the groups."""
+    n = means[0].shape[0]
+    G = n // block
+    th = []
+    for g in range(G):
+        keep = np.r_[0:g * block, (g + 1) * block:n]
+        c = analysis.estimators_from_time_means(*[a[keep] for a in means], L)
+        th.append([c[0], c[2], c[4], c[6]])
+    th = np.asarray(th)
+    return np.sqrt((G - 1) / G * ((th - th.mean(axis=0)) ** 2).sum(axis=0))
+
+
 def compare(ours, ref):
+    """z = (ours - ref) / hypot(sigma_ours, sigma_ref) per temperature for
+    chi0, xi_L/L, e, m, with the table's own errors and, where the jackknife
+    file exists, with the lane-group jackknife errors for ours (zb)."""
     a, b = np.loadtxt(ours), np.loadtxt(ref)
-    print(" beta      z(chi0)  z(xi/L)  z(e)    z(m)")
+    jk = np.loadtxt(jk_path(ours)) if os.path.exists(jk_path(ours)) else None
+    print(" beta      z(chi0)  z(xi/L)  z(e)    z(m)   |  zb(chi0) zb(xi/L) zb(e)   zb(m)")
     z = np.zeros((len(a), 4))
+    zb = np.zeros((len(a), 4))
     for t in range(len(a)):
         for k, c in enumerate((1, 3, 5, 7)):
             s = np.hypot(a[t, c + 1], b[t, c + 1])
             z[t, k] = (a[t, c] - b[t, c]) / s if s > 0 else 0.0
-        print(f" {a[t, 0]:.4f}  " + "  ".join(f"{v:+6.2f}" for v in z[t]))
-    return z
+            if jk is not None:
+                sb = np.hypot(max(jk[t, 1 + k], a[t, c + 1]), b[t, c + 1])
+                zb[t, k] = (a[t, c] - b[t, c]) / sb if sb > 0 else 0.0
+        line = f" {a[t, 0]:.4f}  " + "  ".join(f"{v:+6.2f}" for v in z[t])
+        if jk is not None:
+            line += "  |  " + "  ".join(f"{v:+6.2f}" for v in zb[t])
+        print(line)
+    print(f"max |z| = {np.abs(z).max():.2f}" +
+          (f", max |zb| = {np.abs(zb).max():.2f}" if jk is not None else ""))
+    return z, zb
 
 
 def main():

@@ -100,3 +100,25 @@ def test_obs_records_from_integer_fields():
     for d, c in enumerate((x, y, z)):
         assert abs(o["q_1_r"][d] - (q * np.cos(k * c)).sum() / N) < 1e-6
         assert abs(o["q_1_i"][d] - (q * np.sin(k * c)).sum() / N) < 1e-6
+
+
+def test_group_jackknife_errors():
+    """The lane-group jackknife agrees with the per-sample formula when
+    samples are independent and picks up a fluctuation shared by the 64
+    samples of a group, which the per-sample formula misses."""
+    import reproduce_tables
+    rng = np.random.default_rng(7)
+    S, T, L = 8192, 2, 8
+    means = [rng.random((S, T)) + 1.0, rng.random((S, T, 3)) * 0.1 + 0.05,
+             rng.random((S, T, 3)) * 0.1 + 0.05, -rng.random((S, T)), rng.random((S, T)) * 0.01]
+    jk = reproduce_tables.group_jackknife_errors(means, L)
+    naive = analysis.estimators_from_time_means(*means, L)
+    for k, col in enumerate((1, 3, 5, 7)):
+        np.testing.assert_allclose(jk[k], naive[col], rtol=0.35)
+    shared = np.repeat(rng.normal(0.0, 0.05, (S // 64, T)), 64, axis=0)
+    means[3] = means[3] + shared
+    jk = reproduce_tables.group_jackknife_errors(means, L)
+    naive = analysis.estimators_from_time_means(*means, L)
+    expected = math.sqrt((1 / 12) / S + 0.05 ** 2 / (S // 64))
+    np.testing.assert_allclose(jk[2], expected, rtol=0.35)
+    assert np.all(jk[2] > 1.4 * naive[5])

@@ -125,3 +125,27 @@ def test_reference_protocol_l16_matches_published_table(H):
     hot = ref[:, 0] <= 0.68
     assert np.all(np.abs(z_e[hot]) < 5) and np.all(np.abs(z_q[hot]) < 5)
     assert np.all(z_e[~hot] > -5) and np.all(z_q[~hot] < 5)
+
+
+@pytest.mark.parametrize("L,H", [(16, "0.0000"), (16, "0.0625"), (16, "0.1250"), (16, "0.1875"),
+                                 (12, "0.0000"), (14, "0.0000")])
+def test_reproduced_tables_agree_with_the_published_ones(L, H):
+    """The reference's six published tables against this engine's run of the
+    same protocol (reproduce_tables.py: 2048 samples, 2 copies, its 24
+    temperatures, 4,194,304 sweeps, the second half analysed), committed as
+    fixtures with lane-group jackknife errors.  Every temperature and column
+    (chi0, xi_L/L, e, m): |z| < 3 with z = (ours - ref) / hypot(sigma_ours,
+    sigma_ref), sigma_ours the larger of the table formula and the jackknife
+    over 64-sample lane groups (their samples share each update's random
+    number).  Each table also reads the same temperatures and the same
+    header as the reference's."""
+    import reproduce_tables
+    here = os.path.dirname(os.path.abspath(__file__))
+    ours = os.path.join(here, "golden", "engine", str(L), f"{H}-res.dat")
+    ref = os.path.join(here, "golden", "res", str(L), f"{H}-res.dat")
+    a, b = np.loadtxt(ours), np.loadtxt(ref)
+    np.testing.assert_allclose(a[:, 0], b[:, 0], rtol=1e-11)
+    with open(ours) as f, open(ref) as g:
+        assert f.readline() == g.readline()
+    z, zb = reproduce_tables.compare(ours, ref)
+    assert np.abs(zb).max() < 3.0
