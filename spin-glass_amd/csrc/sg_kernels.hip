@@ -596,6 +596,9 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
 #pragma unroll
         for (int k = 2; k < 6; ++k) jw[q][c][k] = r[k];
       }
+      // the two colours' sites of index h are x neighbours: one bond, one
+      // register pair
+      jw[q][1][0] = jw[q][0][0];
     }
   }
 
