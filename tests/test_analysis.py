@@ -122,3 +122,18 @@ def test_group_jackknife_errors():
     expected = math.sqrt((1 / 12) / S + 0.05 ** 2 / (S // 64))
     np.testing.assert_allclose(jk[2], expected, rtol=0.35)
     assert np.all(jk[2] > 1.4 * naive[5])
+
+
+def test_plots_of_the_reproduced_tables(tmp_path):
+    """The figure set of the reference's make-plots.py (L = 16 at H = 0, the
+    H series with m/H, the L series), drawn from this engine's reproduced
+    tables with the published ones overlaid."""
+    import plots
+    here = os.path.dirname(os.path.abspath(__file__))
+    paths = plots.make_plots(os.path.join(here, "golden", "engine"), str(tmp_path),
+                             overlay=os.path.join(here, "golden", "res"), fmt="png")
+    names = sorted(os.path.basename(p) for p in paths)
+    expect = sorted([f"{k}-{v}.png" for k in ("0", "H", "L") for v in ("chi_0", "xi_LL", "e", "m")]
+                    + ["H-mH.png"])
+    assert names == expect
+    assert all(os.path.getsize(p) > 1000 for p in paths)
