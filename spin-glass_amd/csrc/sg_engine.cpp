@@ -505,9 +505,16 @@ struct sg_batch {
     for (const auto &g : c.graphs)
       if (g.k == k && g.count == count && g.emode == e_last) exec = g.exec;
     if (!exec) {
-      hipGraph_t g;
+      hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
-      for (int i = 0; i < count; ++i) launch_class_interval(c, k ^ (i & 1), i + 1 < count ? e_mid : e_last);
+      try {
+        for (int i = 0; i < count; ++i)
+          launch_class_interval(c, k ^ (i & 1), i + 1 < count ? e_mid : e_last);
+      } catch (...) {
+        // leave the stream out of capture mode before reporting
+        if (hipStreamEndCapture(c.st, &g) == hipSuccess && g) hipGraphDestroy(g);
+        throw;
+      }
       HIPCHK(hipStreamEndCapture(c.st, &g));
       const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
       hipGraphDestroy(g);

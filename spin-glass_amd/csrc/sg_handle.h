@@ -863,9 +863,15 @@ struct sg_handle {
       if (k.parity == par && k.seg == seg && k.energy == want_energy && k.count == count)
         exec = k.exec;
     if (!exec) {
-      hipGraph_t g;
+      hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      for (int i = 0; i < count; ++i) launch_interval((par + i) & 1, 1, seg, want_energy);
+      try {
+        for (int i = 0; i < count; ++i) launch_interval((par + i) & 1, 1, seg, want_energy);
+      } catch (...) {
+        // leave the stream out of capture mode before reporting
+        if (hipStreamEndCapture(st, &g) == hipSuccess && g) hipGraphDestroy(g);
+        throw;
+      }
       HIPCHK(hipStreamEndCapture(st, &g));
       hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
       hipGraphDestroy(g);
