@@ -132,10 +132,11 @@ def main():
                          "instantiated before the timed region)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", default="l16", choices=["l16", "mixed", "large", "serve"])
-    ap.add_argument("--sweeps", type=int, default=0, help="sweeps per job (mixed, large)")
+    ap.add_argument("--workload", default="l16", choices=["l16", "mixed", "large", "serve", "field"])
+    ap.add_argument("--sweeps", type=int, default=0, help="sweeps per job (mixed, large, field)")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
     ap.add_argument("--jobs", type=int, default=400, help="serve: number of jobs")
+    ap.add_argument("--field-h", type=float, default=0.0625, help="field: H")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -183,5 +183,31 @@ def serve(args):
                  steps=args.jobs)
 
 
+# ------------------------------------------------------------------ field
+def field(args):
+    """The default line's shape at H != 0 (the reference publishes L = 16 at
+    H = 1/16, 2/16, 3/16): L=16, 2048 samples, its 24-temperature ladder,
+    2 replicas, PT every 16, batched form."""
+    H = args.field_h
+    sweeps = args.sweeps or 1024 * 4
+    sweeps -= sweeps % PT_EVERY
+    s = sg.Sim(L=16, n_samples=2048, n_replicas=2, betas=sg.REF_BETAS, H=H, pt_every=PT_EVERY,
+               meas_every=1024, form=sg.SG_FORM_BATCHED)
+    s.keep_records = False
+    s.init_lattice()
+    s.run(64 * PT_EVERY)
+    t0 = time.perf_counter()
+    s.run(sweeps)
+    dt = time.perf_counter() - t0
+    s.close()
+    flips = 16.0 ** 3 * 64 * 32 * 2 * 24 * sweeps
+    value = flips / dt / 1e9
+    cfg = {"workload": f"L=16, 2048 samples, the reference's 24 temperatures, 2 replicas, H={H}, "
+                       f"PT every {PT_EVERY}, measurement every 1024, {sweeps} sweeps",
+           "form": "batched", "parallelism": "1 GPU"}
+    return _base(f"spin flips/ns, L=16 at H={H}", value, "spin flips/ns", dt, cfg,
+                 _valu_roofline(value, "k_block<16,true>"), steps=sweeps // PT_EVERY)
+
+
 def run(args):
-    return {"mixed": mixed, "large": large, "serve": serve}[args.workload](args)
+    return {"mixed": mixed, "large": large, "serve": serve, "field": field}[args.workload](args)
