@@ -32,8 +32,9 @@ def test_library_exports_every_declared_function():
 
 def test_create_rejects_bad_parameters_without_device():
     # parameter validation happens before any HIP call
-    for kw in (dict(L=7), dict(L=0), dict(L=50), dict(n_samples=0), dict(H=4.5),
-               dict(n_replicas=0), dict(betas=[0.5, -1.0])):
+    for kw in (dict(L=7), dict(L=0), dict(L=50), dict(n_samples=0), dict(H=4.5), dict(H=-0.1),
+               dict(n_replicas=0), dict(n_replicas=9), dict(betas=[0.5, -1.0]),
+               dict(betas=[1.0] * 4097), dict(kernel=4), dict(form=2)):
         args = dict(L=8, n_samples=64, betas=[1.0, 0.5])
         args.update(kw)
         try:

@@ -191,6 +191,20 @@ def test_large_lattice_field_matches_oracle(L):
     np.testing.assert_array_equal(sim.records[0][1], orc.meas(0))
 
 
+@pytest.mark.parametrize("form", [sg.SG_FORM_PLAIN, sg.SG_FORM_BATCHED])
+def test_maximum_replicas_and_many_temperatures_match_oracle(form):
+    """The largest replica count accepted (8), 40 temperatures (19 and 20 PT
+    pairs by parity), a partial second lane group, PT every sweep."""
+    betas = geometric_betas(0.3, 2.0, 40)
+    sim, orc = _pair(6, 70, 8, betas, pt_every=1, meas_every=2, form=form)
+    for n in (3, 4):
+        sim.run(n)
+        orc.run(n)
+        _assert_same(sim, orc)
+    for i, (_, rec) in enumerate(sim.records):
+        np.testing.assert_array_equal(rec, orc.meas(i))
+
+
 @pytest.mark.parametrize("H", [0.0625, 0.5, 2.0, 2.5, 4.0])
 def test_field_matches_oracle(H):
     """H != 0 thresholds by spin bit and unsatisfied bonds u: at H = 2 the
