@@ -84,7 +84,10 @@ typedef void (*sg_obs_sink)(void *user, int64_t sweep, const int32_t *rec,
                             int64_t n_records, int32_t nfield);
 
 /* replaces the reference's constants and ladder, eamsim.cuh:13-25 and
- * eamsim::init_repib (eamsim.cu:725-744) */
+ * eamsim::init_repib (eamsim.cu:725-744): L = 16, 2048 samples, 2 copies,
+ * the 24-temperature ladder, PT every 16, a record every 1024, the batched
+ * form (every group in one launch per PT interval; SG_FORM_PLAIN gives the
+ * same bits one group at a time) and SG_KERNEL_AUTO */
 void sg_params_default(sg_params *p);
 
 /* replaces eamsim::eamsim(float H) (eamsim.cu:516-543): validates, allocates

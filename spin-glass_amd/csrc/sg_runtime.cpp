@@ -99,7 +99,7 @@ void sg_params_default(sg_params *p) {
   p->L = 16; p->n_samples = 2048; p->n_temps = 24; p->n_replicas = 2;
   p->betas = ref_betas; p->H = 0.0;
   p->coupling_seed = 0x5EED0001ull; p->spin_seed = 0x5EED0002ull;
-  p->pt_every = 16; p->meas_every = 1024; p->device = 0; p->form = SG_FORM_PLAIN;
+  p->pt_every = 16; p->meas_every = 1024; p->device = 0; p->form = SG_FORM_BATCHED;
 }
 
 sg_status sg_create(const sg_params *p, sg_handle **out) {
