@@ -339,15 +339,21 @@ __device__ __forceinline__ void dpp_add32(uint32_t (&A)[NA]) {
 //     one integer per lane per tree wave, which wave 0 adds.
 // fold holds the first PS of the NL*P0 planes (NT words each) and fold2 the
 // rest; red holds TW*NL*64 int32; all LDS.  Every thread calls it; out is
-// valid in wave 0 (lane = sample).
+// valid in wave 0 (lane = sample).  `during` runs in every thread between the
+// two barriers, after the tree in the tree waves: work that touches neither
+// fold nor red (a lattice's write-out) overlaps the tree there.
+struct NoWork {
+  __device__ void operator()() const {}
+};
 template <int NT, int PS>
 __device__ __forceinline__ uint64_t *fold_plane(uint64_t *fold, uint64_t *fold2, int idx) {
   return idx < PS ? fold + idx * NT : fold2 + (idx - PS) * NT;
 }
 
-template <int P0, int NT, int NL, int PS = NL * P0>
+template <int P0, int NT, int NL, int PS = NL * P0, class Fn = NoWork>
 __device__ void wg_count_lanes(const uint64_t (&A)[NL][P0], uint64_t *fold, int32_t *red,
-                               int32_t (&out)[NL], uint64_t *fold2 = nullptr) {
+                               int32_t (&out)[NL], uint64_t *fold2 = nullptr,
+                               const Fn &during = Fn()) {
   constexpr int NW = NT / 64;
   constexpr int TW = NW < 4 ? NW : 4;
   constexpr int FT = TW * 64;
@@ -417,6 +423,7 @@ __device__ void wg_count_lanes(const uint64_t (&A)[NL][P0], uint64_t *fold, int3
       red[(wave * NL + a) * 64 + lane] = v;
     }
   }
+  during();
   __syncthreads();
 #pragma unroll
   for (int a = 0; a < NL; ++a) {
@@ -716,7 +723,9 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
     }
   }
 
-  {
+  // the lattices leave for global memory; with energies, while the tree
+  // waves of the reduction run (the write-out reads only the lattices)
+  auto write_out = [&]() {
     SG_GLOBAL uint64_t *g0 = spins + d0.spin_off, *g1 = spins + d1.spin_off;
     for (int i = tid; i < G::NH; i += G::NT) {
       g0[i] = latB[0][i];
@@ -727,11 +736,12 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
         g1[i] = latB[1][i];
         g1[G::NH + i] = latW[1][i];
       }
-  }
+  };
+  if (!want_energy) write_out();
 
   if (want_energy) {
     int32_t U[2];
-    wg_count_lanes<4, G::NT, 2>(AU, fold, redi, U);
+    wg_count_lanes<4, G::NT, 2>(AU, fold, redi, U, nullptr, write_out);
     if (tid < 64) {
       energy[(size_t)d0.e_idx * 128 + tid] = U[0];
       if (two) energy[(size_t)d1.e_idx * 128 + tid] = U[1];
