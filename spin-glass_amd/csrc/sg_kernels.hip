@@ -102,6 +102,51 @@ __device__ __forceinline__ u32x4 philox_dev(uint32_t c0, uint32_t c1, uint32_t c
   return r;
 }
 
+// The same numbers for a counter (j, n, g, w) in which only j differs
+// between the threads: the first round's product of g and the second round's
+// product of the first output word (both the same in every thread) are
+// taken as 32-bit scalar multiplies, off the vector unit.
+__device__ __forceinline__ u32x4 philox_sweep(uint32_t j, uint32_t n, uint32_t g, uint32_t w,
+                                              uint32_t k0, uint32_t k1) {
+  // round 1: M1 g is uniform, M0 j depends on the thread only (not the sweep)
+  const uint64_t p0 = (uint64_t)0xD2511F53u * j;
+  uint32_t c0 = __umulhi(0xCD9E8D57u, g) ^ n ^ k0;
+  uint32_t c1 = 0xCD9E8D57u * g;
+  uint32_t c2 = SG_BITOP3((uint32_t)(p0 >> 32), w, k1, kXor3);
+  uint32_t c3 = (uint32_t)p0;
+  k0 += 0x9E3779B9u;
+  k1 += 0xBB67AE85u;
+  // round 2: c0 is uniform
+  {
+    const uint32_t q0hi = __umulhi(0xD2511F53u, c0), q0lo = 0xD2511F53u * c0;
+    const uint64_t q1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = SG_BITOP3((uint32_t)(q1 >> 32), c1, k0, kXor3);
+    const uint32_t n2 = SG_BITOP3(q0hi, c3, k1, kXor3);
+    c1 = (uint32_t)q1;
+    c3 = q0lo;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+#pragma unroll
+  for (int round = 2; round < 10; ++round) {
+    const uint64_t p0r = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1r = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = SG_BITOP3((uint32_t)(p1r >> 32), c1, k0, kXor3);
+    const uint32_t n2 = SG_BITOP3((uint32_t)(p0r >> 32), c3, k1, kXor3);
+    c1 = (uint32_t)p1r;
+    c3 = (uint32_t)p0r;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  u32x4 r;
+  r.v[0] = c0; r.v[1] = c1; r.v[2] = c2; r.v[3] = c3;
+  return r;
+}
+
 // Word i & 3 of a Philox block by selects: an indexed read of the block would
 // place it in memory (LDS or scratch).
 __device__ __forceinline__ uint32_t pick_word(const u32x4 &r, int i) {
@@ -659,9 +704,9 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   const uint32_t sw1 = (kTagMC << 28) | (d1.rt & 0x0FFFFFFFu);
   for (int it = 0; it < nsweeps; ++it) {
     const uint32_t n = n0 + (uint32_t)it;
-    const u32x4 r0 = philox_dev((uint32_t)tid, n, d0.g, sw0, d0.key0, d0.key1);
+    const u32x4 r0 = philox_sweep((uint32_t)tid, n, d0.g, sw0, d0.key0, d0.key1);
     u32x4 r1 = r0;
-    if (two) r1 = philox_dev((uint32_t)tid, n, d1.g, sw1, d1.key0, d1.key1);
+    if (two) r1 = philox_sweep((uint32_t)tid, n, d1.g, sw1, d1.key0, d1.key1);
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       if (active) {
@@ -876,9 +921,9 @@ __global__ __launch_bounds__(1024) void k_big(
         // out of the sweep loop and spill them
         int jq = jq0;
         asm volatile("" : "+v"(jq));
-        const u32x4 r0 = philox_dev((uint32_t)jq, n, d0.g, sw[0], d0.key0, d0.key1);
+        const u32x4 r0 = philox_sweep((uint32_t)jq, n, d0.g, sw[0], d0.key0, d0.key1);
         u32x4 r1 = r0;
-        if (two) r1 = philox_dev((uint32_t)jq, n, d1.g, sw[1], d1.key0, d1.key1);
+        if (two) r1 = philox_sweep((uint32_t)jq, n, d1.g, sw[1], d1.key0, d1.key1);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           const int h = jq + q * G::Q;
@@ -1154,9 +1199,9 @@ __global__ __launch_bounds__(MidGeo<L>::NT) void k_mid(
             uint32_t rw[2];
             if (c == 0 && s == 0) {
               const uint32_t j = (uint32_t)(tid + q * G::NT);
-              const u32x4 ra = philox_dev(j, n, d0.g, sw0, d0.key0, d0.key1);
+              const u32x4 ra = philox_sweep(j, n, d0.g, sw0, d0.key0, d0.key1);
               u32x4 rb = ra;
-              if (two) rb = philox_dev(j, n, d1.g, sw1, d1.key0, d1.key1);
+              if (two) rb = philox_sweep(j, n, d1.g, sw1, d1.key0, d1.key1);
               rw[0] = ra.v[0]; r1[0] = ra.v[1]; r23[q][0][0] = ra.v[2]; r23[q][0][1] = ra.v[3];
               rw[1] = rb.v[0]; r1[1] = rb.v[1]; r23[q][1][0] = rb.v[2]; r23[q][1][1] = rb.v[3];
             } else if (c == 0) {
@@ -1403,7 +1448,7 @@ __global__ __launch_bounds__(OneGeo<L>::NT) void k_one(const JobPtrs *__restrict
           const int j = jb + k * G::PL;
           uint32_t rw[2];
           if (c == 0) {
-            const u32x4 r = philox_dev((uint32_t)j, n, d.g, sw, d.key0, d.key1);
+            const u32x4 r = philox_sweep((uint32_t)j, n, d.g, sw, d.key0, d.key1);
             rw[0] = r.v[0];
             rw[1] = r.v[1];
             r23[k][0] = r.v[2];
@@ -1754,7 +1799,7 @@ __global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ 
     if (active) {
       const uint32_t n = ld_uniform(jp.js).sweep + (uint32_t)it;
       const u32x4 r =
-          philox_dev((uint32_t)j, n, d.g, (kTagMC << 28) | (d.rt & 0x0FFFFFFFu), d.key0, d.key1);
+          philox_sweep((uint32_t)j, n, d.g, (kTagMC << 28) | (d.rt & 0x0FFFFFFFu), d.key0, d.key1);
       rw[0] = c ? r.v[2] : r.v[0];
       rw[1] = c ? r.v[3] : r.v[1];
     }
@@ -1914,7 +1959,7 @@ __global__ __launch_bounds__(SlabGeo<L>::NT) void k_slab(const JobPtrs *__restri
 #pragma unroll
     for (int i = 0; i < G::CPT; ++i) {
       const int col = cols[i];
-      const u32x4 r = philox_dev((uint32_t)(col + G::PL * z0), n, d.g, sw, d.key0, d.key1);
+      const u32x4 r = philox_sweep((uint32_t)(col + G::PL * z0), n, d.g, sw, d.key0, d.key1);
       if (e >= 1 && e <= G::S) {
         r23[e - 1][i][0] = r.v[2];
         r23[e - 1][i][1] = r.v[3];
