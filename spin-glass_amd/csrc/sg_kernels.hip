@@ -704,9 +704,7 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
   const uint32_t sw1 = (kTagMC << 28) | (d1.rt & 0x0FFFFFFFu);
   for (int it = 0; it < nsweeps; ++it) {
     const uint32_t n = n0 + (uint32_t)it;
-    const u32x4 r0 = philox_sweep((uint32_t)tid, n, d0.g, sw0, d0.key0, d0.key1);
-    u32x4 r1 = r0;
-    if (two) r1 = philox_sweep((uint32_t)tid, n, d1.g, sw1, d1.key0, d1.key1);
+    u32x4 rb[2];   // the sweep's Philox block of each lattice
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       if (active) {
@@ -716,7 +714,6 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
           uint64_t *mine = c == 0 ? latB[a] : latW[a];
           const uint64_t *opp = c == 0 ? latW[a] : latB[a];
           const ThrRec &tr = a == 0 ? tr0 : tr1;
-          const u32x4 &r = a == 0 ? r0 : r1;
 #pragma unroll
           for (int q = 0; q < 2; ++q) {
             uint64_t nv[6];
@@ -726,8 +723,14 @@ __global__ __launch_bounds__(Geo<L>::NT) void k_block(
             nv[3] = opp[yz[q][1]];
             nv[4] = opp[yz[q][2]];
             nv[5] = opp[yz[q][3]];
+            // at colour 0 the lattice's Philox block is drawn while the first
+            // site's neighbour reads are in flight
+            if (c == 0 && q == 0) {
+              const LatDesc &d = a == 0 ? d0 : d1;
+              rb[a] = philox_sweep((uint32_t)tid, n, d.g, a == 0 ? sw0 : sw1, d.key0, d.key1);
+            }
             uint64_t s = own[a][c][q];
-            update_site<L, FIELD>(s, nv, jw[q][c], r.v[2 * c + q], tr);
+            update_site<L, FIELD>(s, nv, jw[q][c], rb[a].v[2 * c + q], tr);
             mine[tid + q * G::Q] = s;
             own[a][c][q] = s;
           }
