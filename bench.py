@@ -162,6 +162,7 @@ def main():
         p = time_form(SAMPLES_PLAIN, sg.SG_FORM_PLAIN, args.steps, args.warmup)
         plain = p["flips"] / p["wall_s"] / 1e9
     avg_ms = k["sweep_kernel_ms"] / max(k["launches"], 1)
+    hbm_gbs = TRAFFIC_L16 / (avg_ms * 1e6) if avg_ms > 0 else 0.0
     flips_per_launch = k["sweep_kernel_flips"] / max(k["launches"], 1)
     achieved = W_ALG * flips_per_launch / (avg_ms * 1e-3) / 1e12
     line = {
@@ -207,6 +208,10 @@ def main():
                       f"steps (direct launches); value and ms_per_step come from the first "
                       f"pass, which replays the captured intervals without events",
             "traffic": TRAFFIC_L16,
+            # the same launch against HBM: counter-measured bytes per launch
+            # over the event-timed launch duration
+            "hbm": {"achieved": round(hbm_gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                    "frac": round(hbm_gbs / 8000.0, 4)},
         },
     }
     if not args.no_cpu:
