@@ -243,6 +243,23 @@ def test_reference_ladder_l16_matches_oracle():
     np.testing.assert_array_equal(sim.records[0][1], orc.meas(0))
 
 
+def test_benchmark_workload_at_full_size_matches_oracle():
+    """The benchmark's default workload at its full size: L=16, 2048 samples
+    (32 groups), 24 temperatures geometric in T = [0.2, 2], 2 replicas, PT
+    every 16 sweeps, batched form with the tuner's work-group shape.  Three PT
+    intervals and one measurement: every spin word, per-lane energy, swap
+    count and record equal to the oracle's, FP64 results within 1e-12."""
+    betas = geometric_betas(0.2, 2.0, 24)
+    sim, orc = _pair(16, 2048, 2, betas, pt_every=16, meas_every=32, form=sg.SG_FORM_BATCHED)
+    sim.run(48)
+    orc.run(48)
+    _assert_same(sim, orc)
+    assert len(sim.records) == orc.n_meas() == 1
+    np.testing.assert_array_equal(sim.records[0][1], orc.meas(0))
+    ref = results_from_records([orc.meas(0)], 16, 2, 0.0, 2048, betas)
+    np.testing.assert_allclose(sim.results(), ref, rtol=1e-12, atol=0)
+
+
 def test_checkpoint_resume_is_bit_exact():
     betas = geometric_betas(0.5, 2.0, 4)
     a = sg.Sim(L=8, n_samples=64, betas=betas, pt_every=3, meas_every=5)
