@@ -223,3 +223,38 @@ def test_autotuner_measures_every_shape_and_caches():
     assert k == min((P, O, S), key=lambda m: ms[m])
     assert sg.tune_shapes(32, 60) == (k, ms)                     # same power-of-two bucket
     assert sg.tune_kernel(32, 64) == (k, ms[P], ms[S])
+
+
+@pytest.mark.gpu
+def test_mixed_benchmark_batch_at_full_size_matches_oracle():
+    """The mixed-L benchmark batch at its full size: eight 64-sample jobs with
+    L per group [20, 8, 20, 8, 20, 12, 20, 20] (its seeded draw), 32
+    temperatures geometric in T = [0.2, 2], 2 replicas, PT every 16, in one
+    batched engine (three classes on three streams, most intervals replayed
+    from captured graphs).  Every job's FP64 results and swap counts equal
+    the oracle's run of the same job.  Results: 1e-12 relative, with an
+    absolute floor of 1e-13 for means that nearly cancel (m, about 7e-4 from
+    terms of order 1: 256 such terms summed in another order differ by up to
+    256 * 2^-53 = 3e-14 of the largest)."""
+    Ls = [20, 8, 20, 8, 20, 12, 20, 20]
+    betas = geometric_betas(0.2, 2.0, 32)
+    sweeps, meas = 256, 64
+    jobs = [sg.Job(L=L, n_sweeps=sweeps, n_samples=64, betas=betas,
+                   coupling_seed=0x5EED0001 + g, spin_seed=0x5EED1001 + g, meas_every=meas)
+            for g, L in enumerate(Ls)]
+    b = sg.Batch(pt_every=16)
+    ids = [b.submit(j) for j in jobs]
+    b.drain()
+    st = b.launch_stats()
+    assert st["intervals_graphed"] > 0 and st["intervals_direct"] > 0
+    for j, i in zip(jobs, ids):
+        res, acc = b.result(i)
+        orc = oracle.OracleSim(j.L, 1, 2, betas, coupling_seed=j.coupling_seed,
+                               spin_seed=j.spin_seed, pt_every=16, meas_every=meas)
+        orc.init()
+        orc.run(sweeps)
+        ref = results_from_records([orc.meas(k) for k in range(orc.n_meas())], j.L, 2, 0.0, 64,
+                                   betas)
+        np.testing.assert_allclose(res, ref, rtol=1e-12, atol=1e-13, err_msg=f"L={j.L}")
+        np.testing.assert_array_equal(acc, orc.swap_counts()[0], err_msg=f"L={j.L}")
+    b.close()
