@@ -260,6 +260,23 @@ def test_benchmark_workload_at_full_size_matches_oracle():
     np.testing.assert_allclose(sim.results(), ref, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("L", [32, 48])
+def test_large_lattice_benchmark_at_full_size_matches_oracle(L):
+    """The large-lattice benchmark's shape at its full size: 64 samples, 32
+    temperatures geometric in T = [0.3, 2], 2 replicas (64 lattices), PT every
+    16, batched form with the tuner's shape (slab sweeps at L = 32, colour
+    launches at L = 48).  Two PT intervals and a measurement, bit-exact."""
+    betas = geometric_betas(0.3, 2.0, 32)
+    sim, orc = _pair(L, 64, 2, betas, pt_every=16, meas_every=32, form=sg.SG_FORM_BATCHED)
+    sim.run(32)
+    orc.run(32)
+    _assert_same(sim, orc)
+    assert len(sim.records) == orc.n_meas() == 1
+    np.testing.assert_array_equal(sim.records[0][1], orc.meas(0))
+    ref = results_from_records([orc.meas(0)], L, 2, 0.0, 64, betas)
+    np.testing.assert_allclose(sim.results(), ref, rtol=1e-12, atol=0)
+
+
 def test_checkpoint_resume_is_bit_exact():
     betas = geometric_betas(0.5, 2.0, 4)
     a = sg.Sim(L=8, n_samples=64, betas=betas, pt_every=3, meas_every=5)
