@@ -26,12 +26,15 @@ FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-ffp-contract=
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}"]
 
-LIB_SOURCES = ["sg_kernels.hip", "sg_runtime.cpp", "sg_engine.cpp"]
+LIB_SOURCES = ["sg_kernels_resident.hip", "sg_kernels.hip", "sg_runtime.cpp", "sg_engine.cpp"]
+# The resident sweep kernels take LLVM's max-ILP machine scheduler (faster for
+# them; slower for the streamed kernels, which keep the default).
+SOURCE_FLAGS = {"sg_kernels_resident.hip": ["-mllvm", "--amdgpu-sched-strategy=max-ilp"]}
 
 
 def _compile(src: str) -> str:
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", out]
+    cmd = [HIPCC, *FLAGS, *SOURCE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", out]
     if src.endswith(".cpp"):
         cmd.insert(1, "-xhip")
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -1682,6 +1682,7 @@ __global__ __launch_bounds__(256) void k_init_spins(uint64_t *__restrict__ spins
   spins[d.spin_off + cb_index<L>(x, y, z)] = (uint64_t)r.v[0] | ((uint64_t)r.v[1] << 32);
 }
 
+#ifndef SG_RESIDENT_PART  // non-template kernels: defined once, in sg_kernels.hip
 __global__ __launch_bounds__(256) void k_init_couplings(uint64_t *__restrict__ J, uint64_t j_off,
                                                         int N, uint32_t g, uint32_t key0,
                                                         uint32_t key1) {
@@ -1696,6 +1697,8 @@ __global__ void k_advance(const JobPtrs *jt, int njobs, int nsweeps) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < njobs) jt[i].js->sweep += (uint32_t)nsweeps;
 }
+
+#endif
 
 // ------------------------------------------------------- coupling site records
 // Site record of colour c, checkerboard index h: the six bond words of that
@@ -2041,6 +2044,7 @@ __global__ __launch_bounds__(SlabGeo<L>::NT) void k_slab(const JobPtrs *__restri
   }
 }
 
+#ifndef SG_RESIDENT_PART  // non-template kernels: defined once, in sg_kernels.hip
 // PT step of streamed lattices: a task is (pair, chunk of kPtWords words);
 // every work-group decides the 64 lanes itself (same energies, same Philox
 // words, so the same mask) and exchanges its chunk; chunk 0 counts.
@@ -2083,11 +2087,17 @@ __global__ __launch_bounds__(256) void k_pt_chunk(const JobPtrs *__restrict__ jt
   if (tk.lat[1] == 0 && tid == 0) atomicAdd(&jp.counters[p.cnt_idx], (unsigned long long)__popcll(m));
 }
 
+#endif
+
 // ---------------------------------------------------------------- dispatch
 #define SG_LDS_SIZES(X) X(2) X(4) X(6) X(8) X(10) X(12) X(14) X(16)
 #define SG_ALL_SIZES(X) SG_LDS_SIZES(X) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) \
   X(34) X(36) X(38) X(40) X(42) X(44) X(46) X(48)
 
+// The resident sweep launchers (and so the only instances of k_block, k_mid,
+// k_big and k_one) are compiled in sg_kernels_resident.hip, which defines
+// SG_RESIDENT_PART and includes this file; everything else is compiled here.
+#ifdef SG_RESIDENT_PART
 bool sweep_kernel_supported(int L) {
   switch (L) {
 #define X(l) case l: return true;
@@ -2160,6 +2170,8 @@ hipError_t launch_single(int L, bool field, const JobPtrs *jt, const TaskDesc *t
       return hipErrorInvalidValue;
   }
 }
+
+#else  // !SG_RESIDENT_PART
 
 hipError_t launch_measure(int L, int R, const uint64_t *spins, const int32_t *energy,
                           const MeasDesc *md, int nmd, int32_t *out, int nfield, ObsAcc *acc,
@@ -2302,5 +2314,7 @@ hipError_t launch_advance(const JobPtrs *jt, int njobs, int nsweeps, hipStream_t
                      nsweeps);
   return hipGetLastError();
 }
+
+#endif  // SG_RESIDENT_PART
 
 }  // namespace sg
