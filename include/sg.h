@@ -112,6 +112,14 @@ sg_status sg_init_lattice(sg_handle *h);
  * accumulators */
 sg_status sg_run(sg_handle *h, int64_t n_sweeps, sg_obs_sink sink, void *user);
 
+/* no counterpart in the reference (whose launches are not captured): builds
+ * every captured launch sequence (hipGraph) that sg_run(h, n_sweeps, ...)
+ * from the current sweep would replay, and runs nothing, so that a timed
+ * sg_run afterwards includes no capture or instantiation.  sg_graph_count
+ * gives the number of instantiated graphs the handle holds (-1 for NULL). */
+sg_status sg_prepare(sg_handle *h, int64_t n_sweeps);
+int32_t sg_graph_count(const sg_handle *h);
+
 /* replaces eamsim::save_checkpoint / load_checkpoint (eamsim.cu:655-676).
  * The image carries a versioned header (L, sizes, ladder, seeds, sweep
  * counter, accumulators) and a checksum; load rejects a size, magic,
@@ -172,6 +180,25 @@ sg_status sg_tune_kernel(int32_t device, int32_t L, int32_t n_lattices, int32_t 
  * L has no such shape) */
 sg_status sg_tune_shapes(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
                          int32_t *kernel, double ms[4]);
+
+/* tuner measurements run in this process so far (each sg_tune_* call for an
+ * unmeasured (device, L, field, bucket), and each sg_create with
+ * SG_KERNEL_AUTO for one).  The batch engine never measures: its classes take
+ * the cached result, or for an unmeasured bucket the rule the measurements
+ * support (pairs from 512 lattices at L <= 16 and 256 at L = 18, 20, single
+ * lattices below; the one-lattice kernel at L = 22..26; streamed above). */
+int64_t sg_tune_count(void);
+
+/* VALU issue-rate probe (no counterpart in the reference; measurement only,
+ * for the roofline of the sweep kernels): one launch that fills every CU
+ * (2048 work-groups of 256 threads) with 16 independent chains per thread of
+ * one instruction — kind 0 v_bitop3_b32, kind 1 v_mad_u64_u32 — for iters
+ * steps (even, >= 2), after one untimed launch.  Returns the lane-operations
+ * the timed launch issued and its HIP-event duration in ms.  Errors:
+ * sg_probe_last_error(). */
+sg_status sg_probe_valu(int32_t device, int32_t kind, int32_t iters, double *lane_ops,
+                        double *ms);
+const char *sg_probe_last_error(void);
 
 /* --- batch and serving engine -------------------------------------------
  * No counterpart in the reference, which runs one fixed-size simulation per

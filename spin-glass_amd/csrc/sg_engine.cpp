@@ -7,7 +7,9 @@
 // class share a HIP stream, and in the batched form one sweep-kernel launch
 // per PT interval covers every task of every running job of the class
 // (TaskDesc.job indexes the class's JobPtrs table).  Classes run on separate
-// streams, so the device overlaps them.  Because each job's random numbers
+// streams, so the device overlaps them.  A class's work-group shape comes
+// from the tuner's cache (or the rule its measurements support): no step runs
+// the tuner.  Because each job's random numbers
 // are keyed by its own seeds and counters, the mix never changes a result.
 #include <chrono>
 #include <deque>
@@ -69,7 +71,9 @@ struct DevList {
 // flipped (the lists alternate from one PT interval to the next).  With
 // SG_KERNEL_AUTO the class picks its work-group shape (lattice pairs, single
 // lattices or streamed chunks) whenever its membership changes, from the
-// autotuner's measurement for its lattice count.
+// autotuner's cached measurement for its lattice count (sg_tune_shapes before
+// serving fills the cache) or, for a count it has not measured, the rule its
+// measurements support.
 struct Class {
   int L = 0;
   int kernel = SG_KERNEL_AUTO;
@@ -372,7 +376,7 @@ struct sg_batch {
     if (mode == SG_KERNEL_AUTO) {
       size_t nlat = 0;
       for (Job *j : c.jobs) nlat += j->h->n_lat();
-      mode = tune_kernel_choice(bp.device, c.L, c.field, nlat);
+      mode = cached_kernel_choice(bp.device, c.L, c.field, nlat);
     }
     c.streamed = mode == SG_KERNEL_STREAMED;
     c.single = mode == SG_KERNEL_RESIDENT_SINGLE && !single_kernel_supported(c.L);

@@ -144,6 +144,11 @@ inline uint64_t fnv1a(const void *data, size_t n, uint64_t h = 14695981039346656
 // L, field, power-of-two lattice count).  ms[k] receives the measured time of
 // shape k (0 where the L has no such shape).
 int tune_kernel_choice(int device, int L, bool field, size_t n_lattices, double *ms = nullptr);
+// The same decision without measuring: the cached result when the tuner has
+// measured that (device, L, field, bucket), else the rule its measurements
+// support.  The batch engine decides with this, so no sg_step ever runs the
+// tuner.
+int cached_kernel_choice(int device, int L, bool field, size_t n_lattices);
 
 // Device bytes one engine job needs, block by block as sg_handle::create
 // allocates them from an arena (each block rounded to the arena's
@@ -856,29 +861,28 @@ struct sg_handle {
   // Batched form: `count` consecutive full PT intervals (parities par,
   // 1 - par, ...) are one hipGraph per (parity, length, energy, count),
   // replayed while the device sweep counter advances; fewer graph launches,
-  // fewer gaps between them.
-  void run_graph_interval(int par, int seg, int want_energy, int count = 1) {
-    hipGraphExec_t exec = nullptr;
+  // fewer gaps between them.  graph_for captures and instantiates one on
+  // first use (nothing runs during a capture).
+  hipGraphExec_t graph_for(int par, int seg, int want_energy, int count) {
     for (auto &k : graphs)
       if (k.parity == par && k.seg == seg && k.energy == want_energy && k.count == count)
-        exec = k.exec;
-    if (!exec) {
-      hipGraph_t g = nullptr;
-      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      try {
-        for (int i = 0; i < count; ++i) launch_interval((par + i) & 1, 1, seg, want_energy);
-      } catch (...) {
-        // leave the stream out of capture mode before reporting
-        if (hipStreamEndCapture(st, &g) == hipSuccess && g) hipGraphDestroy(g);
-        throw;
-      }
-      HIPCHK(hipStreamEndCapture(st, &g));
-      hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
-      HIPCHK(e);
-      graphs.push_back(GraphKey{par, seg, want_energy, count, exec});
+        return k.exec;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int i = 0; i < count; ++i) launch_interval((par + i) & 1, 1, seg, want_energy);
+    } catch (...) {
+      // leave the stream out of capture mode before reporting
+      if (hipStreamEndCapture(st, &g) == hipSuccess && g) hipGraphDestroy(g);
+      throw;
     }
-    HIPCHK(hipGraphLaunch(exec, st));
+    HIPCHK(hipStreamEndCapture(st, &g));
+    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HIPCHK(e);
+    graphs.push_back(GraphKey{par, seg, want_energy, count, exec});
+    return exec;
   }
 
   // full PT intervals from sweep n that one graph may replay: up to 4, all
@@ -892,43 +896,71 @@ struct sg_handle {
     return room >= 4 ? 4 : room >= 2 ? 2 : 1;
   }
 
+  // One step of the run loop from sweep n with `remaining` sweeps to go:
+  // seg sweeps per interval (up to the next PT step or measurement), count
+  // intervals (several only when replayed from one graph), PT at the start
+  // of each, and the energy mode of the last: U (and K at H != 0) for the
+  // next PT step, U and K when a measurement follows.
+  struct StepPlan {
+    int64_t seg;
+    int par, count, emode;
+    bool pt_now, graphed;
+  };
+  StepPlan plan_step(int64_t n, int64_t remaining) const {
+    const bool pt = pt_every > 0 && T > 1;
+    StepPlan s{};
+    s.seg = remaining;
+    if (pt) s.seg = std::min<int64_t>(s.seg, pt_every - n % pt_every);
+    if (meas_every > 0) s.seg = std::min<int64_t>(s.seg, meas_every - n % meas_every);
+    const bool want_energy = pt && (n + s.seg) % pt_every == 0;
+    s.pt_now = pt && n % pt_every == 0;
+    s.par = s.pt_now ? (int)((n / pt_every) & 1) : 0;
+    s.count = 1;
+    s.graphed = form == SG_FORM_BATCHED && s.pt_now && s.seg == pt_every && !profiling;
+    if (s.graphed) s.count = graph_run_length(n, remaining);
+    const bool meas_end = meas_every > 0 && (n + s.count * s.seg) % meas_every == 0;
+    s.emode = meas_end ? 2 : want_energy ? energy_mode() : 0;
+    return s;
+  }
+
+  // Instantiates every graph that run(n_sweeps) from the current sweep would
+  // replay, and runs nothing: a caller that times run() afterwards times no
+  // capture or instantiation.
+  void prepare(int64_t n_sweeps) {
+    if (!initialized) throw Fail{SG_ERR_STATE, "lattice not initialised"};
+    if (n_sweeps < 0) throw Fail{SG_ERR_INVALID, "negative sweep count"};
+    int64_t n = sweep, remaining = n_sweeps;
+    while (remaining > 0) {
+      const StepPlan s = plan_step(n, remaining);
+      if (s.graphed) graph_for(s.par, (int)s.seg, s.emode, s.count);
+      n += s.count * s.seg;
+      remaining -= s.count * s.seg;
+    }
+  }
+
   void run(int64_t n_sweeps, sg_obs_sink sink, void *user) {
     if (!initialized) throw Fail{SG_ERR_STATE, "lattice not initialised"};
     if (n_sweeps < 0) throw Fail{SG_ERR_INVALID, "negative sweep count"};
     if (sweep + n_sweeps > 0xFFFFFFFFll) throw Fail{SG_ERR_INVALID, "sweep counter overflow"};
-    const bool pt = pt_every > 0 && T > 1;
     HIPCHK(hipEventRecord(ev0, st));
     int64_t remaining = n_sweeps;
     while (remaining > 0) {
       const int64_t n = sweep;
-      int64_t seg = remaining;
-      if (pt) seg = std::min<int64_t>(seg, pt_every - n % pt_every);
-      if (meas_every > 0) seg = std::min<int64_t>(seg, meas_every - n % meas_every);
-      const int64_t next = n + seg;
-      const bool want_energy = pt && next % pt_every == 0;
-      const bool pt_now = pt && n % pt_every == 0;
-      if (pt_now && !energy_valid) launch_energy_all(energy_mode());
-      const int par = pt_now ? (int)((n / pt_every) & 1) : 0;
-      int count = 1;   // PT intervals run by this step
-      const bool graphed = form == SG_FORM_BATCHED && pt_now && seg == pt_every && !profiling;
-      if (graphed) count = graph_run_length(n, remaining);
-      // energies of the final states: U (and K at H != 0) for the next PT
-      // step, U and K when a measurement follows
-      const bool meas_end = meas_every > 0 && (n + count * seg) % meas_every == 0;
-      const int emode = meas_end ? 2 : want_energy ? energy_mode() : 0;
-      if (graphed)
-        run_graph_interval(par, (int)seg, emode, count);
+      const StepPlan s = plan_step(n, remaining);
+      if (s.pt_now && !energy_valid) launch_energy_all(energy_mode());
+      if (s.graphed)
+        HIPCHK(hipGraphLaunch(graph_for(s.par, (int)s.seg, s.emode, s.count), st));
       else
-        launch_interval(par, pt_now ? 1 : 0, (int)seg, emode);
-      for (int i = 0; i < count; ++i) {
-        if (pt_now)
-          for (int t = (par + i) & 1; t + 1 < T; t += 2) swap_att[t] += (uint64_t)G * R * 64;
-        flips += (double)seg * N * 64.0 * G * R * T;
+        launch_interval(s.par, s.pt_now ? 1 : 0, (int)s.seg, s.emode);
+      for (int i = 0; i < s.count; ++i) {
+        if (s.pt_now)
+          for (int t = (s.par + i) & 1; t + 1 < T; t += 2) swap_att[t] += (uint64_t)G * R * 64;
+        flips += (double)s.seg * N * 64.0 * G * R * T;
       }
-      sweep = n + count * seg;
-      energy_valid = emode > 0;
-      energy_k_valid = emode >= 2;
-      remaining -= count * seg;
+      sweep = n + s.count * s.seg;
+      energy_valid = s.emode > 0;
+      energy_k_valid = s.emode >= 2;
+      remaining -= s.count * s.seg;
       // records of earlier measurements whose copies have landed
       if (!rec_queue.empty()) drain_records(sink, user, 0);
       if (meas_every > 0 && sweep % meas_every == 0) measure(sink, user);

@@ -23,8 +23,10 @@ struct TuneRes {
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, bool, size_t>, TuneRes> g_tune;
+int64_t g_tune_runs = 0;   // tune_measure calls in this process
 
 TuneRes tune_measure(int device, int L, bool field, size_t nlat) {
+  ++g_tune_runs;
   sg_params p;
   std::memset(&p, 0, sizeof(p));
   const int T = (int)std::max<size_t>(1, std::min<size_t>(24, nlat / 2));
@@ -52,14 +54,32 @@ TuneRes tune_measure(int device, int L, bool field, size_t nlat) {
   return res;
 }
 
-const TuneRes &tune_lookup(int device, int L, bool field, size_t n_lattices) {
+size_t tune_bucket(size_t n_lattices) {
   size_t bucket = 1;
   while (bucket < n_lattices) bucket <<= 1;
+  return bucket;
+}
+
+const TuneRes &tune_lookup(int device, int L, bool field, size_t n_lattices) {
+  const size_t bucket = tune_bucket(n_lattices);
   std::lock_guard<std::mutex> lock(g_tune_mu);
   const auto key = std::make_tuple(device, L, field, bucket);
   auto it = g_tune.find(key);
   if (it == g_tune.end()) it = g_tune.emplace(key, tune_measure(device, L, field, bucket)).first;
   return it->second;   // entries are never erased
+}
+
+// The shape the tuner's measurements on the MI355X favour, for a (device, L,
+// field, lattice count) it has not measured: two lattices per work-group
+// once there are enough of them for about two work-groups per CU (from 512
+// lattices at L <= 16, 256 at L = 18, 20), one lattice per work-group below;
+// the one-lattice resident kernel at L = 22..26; streamed chunks above, where
+// a lattice no longer fits in LDS.
+int rule_shape(int L, size_t n_lattices) {
+  if (single_kernel_supported(L)) return SG_KERNEL_RESIDENT;
+  if (L >= 28 && phase_kernel_supported(L)) return SG_KERNEL_STREAMED;
+  const size_t pairs_from = L <= 16 ? 512 : 256;
+  return n_lattices >= pairs_from ? SG_KERNEL_RESIDENT : SG_KERNEL_RESIDENT_SINGLE;
 }
 }  // namespace
 
@@ -69,6 +89,12 @@ int tune_kernel_choice(int device, int L, bool field, size_t n_lattices, double 
   const TuneRes &r = tune_lookup(device, L, field, n_lattices);
   if (ms) std::copy(r.ms, r.ms + 4, ms);
   return r.kernel;
+}
+
+int cached_kernel_choice(int device, int L, bool field, size_t n_lattices) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tune.find(std::make_tuple(device, L, field, tune_bucket(n_lattices)));
+  return it != g_tune.end() ? it->second.kernel : rule_shape(L, n_lattices);
 }
 }  // namespace detail
 }  // namespace sg
@@ -151,6 +177,12 @@ sg_status sg_init_lattice(sg_handle *h) {
 sg_status sg_run(sg_handle *h, int64_t n_sweeps, sg_obs_sink sink, void *user) {
   return guarded(h, [&] { h->run(n_sweeps, sink, user); });
 }
+
+sg_status sg_prepare(sg_handle *h, int64_t n_sweeps) {
+  return guarded(h, [&] { h->prepare(n_sweeps); });
+}
+
+int32_t sg_graph_count(const sg_handle *h) { return h ? (int32_t)h->graphs.size() : -1; }
 
 sg_status sg_save_checkpoint(sg_handle *h, void *buf, size_t *size) {
   if (!size) return SG_ERR_INVALID;
@@ -275,6 +307,11 @@ sg_status sg_tune_shapes(int32_t device, int32_t L, int32_t n_lattices, int32_t 
     g_create_error = e.msg;
     return e.code;
   }
+}
+
+int64_t sg_tune_count(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  return g_tune_runs;
 }
 
 sg_status sg_get_kernel_stats(sg_handle *h, int32_t kind, int64_t *launches, double *total_ms,

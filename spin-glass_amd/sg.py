@@ -58,6 +58,8 @@ EXPORTS = [
     "sg_batch_params_default", "sg_batch_create", "sg_batch_destroy", "sg_batch_last_error",
     "sg_job_footprint", "sg_submit", "sg_step", "sg_drain", "sg_poll", "sg_result",
     "sg_job_times", "sg_batch_stats", "sg_batch_launch_stats", "sg_serve",
+    "sg_probe_valu", "sg_probe_last_error", "sg_prepare", "sg_graph_count",
+    "sg_tune_count",
 ]
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
@@ -129,6 +131,12 @@ def load_library(path: str = LIB_PATH):
     L.sg_batch_stats.argtypes = [_vp, _dp, _i64p, _i64p, _i64p]
     L.sg_batch_launch_stats.argtypes = [_vp, _i64p, _i64p, _i64p]
     L.sg_serve.argtypes = [_vp, ctypes.POINTER(_Params), _i64p, _dp, ctypes.c_int32, _i64p, _dp]
+    L.sg_tune_count.argtypes = []
+    L.sg_tune_count.restype = ctypes.c_int64
+    L.sg_prepare.argtypes = [_vp, ctypes.c_int64]
+    L.sg_graph_count.argtypes = [_vp]
+    L.sg_probe_valu.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp]
+    L.sg_probe_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -219,6 +227,15 @@ class Sim:
                 sink(sweep, arr)
         cb = SINK(_cb) if (self.keep_records or sink is not None) else SINK()
         self._c(load_library().sg_run(self._h, int(n_sweeps), cb, None))
+
+    def prepare(self, n_sweeps):
+        """Instantiate every captured launch sequence that run(n_sweeps) would
+        replay from the current sweep, running nothing."""
+        self._c(load_library().sg_prepare(self._h, int(n_sweeps)))
+
+    @property
+    def graph_count(self):
+        return load_library().sg_graph_count(self._h)
 
     def spins(self):
         out = np.zeros((self.G, self.R, self.T, self.N), dtype=np.uint64)
@@ -314,6 +331,23 @@ def tune_shapes(L, n_lattices, field=False, device=0):
     _check(lib.sg_tune_shapes(device, L, n_lattices, 1 if field else 0, ctypes.byref(k), ms))
     return k.value, {SG_KERNEL_RESIDENT: ms[1], SG_KERNEL_STREAMED: ms[2],
                      SG_KERNEL_RESIDENT_SINGLE: ms[3]}
+
+
+def tune_count():
+    """Tuner measurements run in this process so far."""
+    return load_library().sg_tune_count()
+
+
+def probe_valu(kind=0, iters=4096, device=0):
+    """VALU issue-rate probe: (lane-operations, ms, lane-ops per second) of
+    one launch of 16 independent instruction chains per thread over every CU;
+    kind 0 = v_bitop3_b32, 1 = v_mad_u64_u32."""
+    lib = load_library()
+    ops, ms = ctypes.c_double(), ctypes.c_double()
+    st = lib.sg_probe_valu(device, kind, iters, ctypes.byref(ops), ctypes.byref(ms))
+    if st != 0:
+        raise SGError(f"libsg status {st}: {lib.sg_probe_last_error().decode()}")
+    return ops.value, ms.value, ops.value / (ms.value * 1e-3)
 
 
 # ------------------------------------------------------------------ engine
