@@ -79,7 +79,7 @@ SHAPE_NAMES = {sg.SG_KERNEL_RESIDENT: "lattice pairs", sg.SG_KERNEL_RESIDENT_SIN
                sg.SG_KERNEL_STREAMED: "streamed chunks"}
 
 
-CLOCK_WARM_STEPS = 96   # about 16 ms of the timed workload's own launches
+CLOCK_WARM_STEPS = 160  # about 27 ms of the timed workload's own launches
 
 
 def time_form(samples, form, steps, warmup, profile=False):
@@ -102,7 +102,8 @@ def time_form(samples, form, steps, warmup, profile=False):
     s.run(CLOCK_WARM_STEPS * PT_EVERY)
     s.init_lattice()
     s.set_profiling(profile)
-    s.prepare((warmup + steps) * PT_EVERY)
+    s.prepare(warmup * PT_EVERY)
+    s.prepare(steps * PT_EVERY, from_sweep=warmup * PT_EVERY)
     graphs = s.graph_count
     s.run(warmup * PT_EVERY)
     n0, ms0, fl0 = s.kernel_stats(0)

@@ -113,11 +113,12 @@ sg_status sg_init_lattice(sg_handle *h);
 sg_status sg_run(sg_handle *h, int64_t n_sweeps, sg_obs_sink sink, void *user);
 
 /* no counterpart in the reference (whose launches are not captured): builds
- * every captured launch sequence (hipGraph) that sg_run(h, n_sweeps, ...)
- * from the current sweep would replay, and runs nothing, so that a timed
- * sg_run afterwards includes no capture or instantiation.  sg_graph_count
- * gives the number of instantiated graphs the handle holds (-1 for NULL). */
-sg_status sg_prepare(sg_handle *h, int64_t n_sweeps);
+ * every captured launch sequence (hipGraph) that a call sg_run(h, n_sweeps,
+ * ...) starting at sweep from_sweep (the current sweep when negative) would
+ * replay, and runs nothing, so that a timed sg_run afterwards includes no
+ * capture or instantiation.  sg_graph_count gives the number of instantiated
+ * graphs the handle holds (-1 for NULL). */
+sg_status sg_prepare(sg_handle *h, int64_t from_sweep, int64_t n_sweeps);
 int32_t sg_graph_count(const sg_handle *h);
 
 /* replaces eamsim::save_checkpoint / load_checkpoint (eamsim.cu:655-676).

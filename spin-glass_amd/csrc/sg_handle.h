@@ -150,6 +150,31 @@ int tune_kernel_choice(int device, int L, bool field, size_t n_lattices, double 
 // tuner.
 int cached_kernel_choice(int device, int L, bool field, size_t n_lattices);
 
+// Metropolis thresholds of one temperature (see ThrRec): FP64 on the host,
+// floor(2^32 det_exp(-beta dE)) for every (spin bit, u) with dE > 0.
+inline void build_threshold_rec(double b, double H, ThrRec &r) {
+#pragma clang fp contract(off)
+  std::memset(&r, 0, sizeof(r));
+  for (int sig = 0; sig < 2; ++sig) {
+    const double sv = sig ? 1.0 : -1.0;
+    uint32_t nfin = 0;
+    for (int u = 0; u < 7; ++u) {
+      const double dE = (double)(12 - 4 * u) + 2.0 * H * sv;
+      if (dE <= 0.0) break;
+      const double p = det_exp(-b * dE);
+      const uint64_t thr = (uint64_t)(p * 4294967296.0);
+      if (thr >= (1ull << 32)) throw Fail{SG_ERR_INVALID, "threshold overflow"};
+      r.t[sig][u] = (uint32_t)thr;
+      nfin = (uint32_t)(u + 1);
+    }
+    r.nfin[sig] = nfin;
+    // the sweep kernels take u = 5, 6 as always accepted (true for H <= 4)
+    if (nfin > 5) throw Fail{SG_ERR_INVALID, "H out of range [0, 4]"};
+  }
+  if (H == 0.0 && (r.nfin[0] != 3 || r.nfin[1] != 3))
+    throw Fail{SG_ERR_INVALID, "unexpected threshold table at H = 0"};
+}
+
 // Device bytes one engine job needs, block by block as sg_handle::create
 // allocates them from an arena (each block rounded to the arena's
 // alignment; no record buffer).
@@ -692,28 +717,7 @@ struct sg_handle {
     }
   }
 
-  void build_thresholds(double b, ThrRec &r) const {
-#pragma clang fp contract(off)
-    std::memset(&r, 0, sizeof(r));
-    for (int sig = 0; sig < 2; ++sig) {
-      const double sv = sig ? 1.0 : -1.0;
-      uint32_t nfin = 0;
-      for (int u = 0; u < 7; ++u) {
-        const double dE = (double)(12 - 4 * u) + 2.0 * H * sv;
-        if (dE <= 0.0) break;
-        const double p = det_exp(-b * dE);
-        const uint64_t thr = (uint64_t)(p * 4294967296.0);
-        if (thr >= (1ull << 32)) throw Fail{SG_ERR_INVALID, "threshold overflow"};
-        r.t[sig][u] = (uint32_t)thr;
-        nfin = (uint32_t)(u + 1);
-      }
-      r.nfin[sig] = nfin;
-      // the sweep kernels take u = 5, 6 as always accepted (true for H <= 4)
-      if (nfin > 5) throw Fail{SG_ERR_INVALID, "H out of range [0, 4]"};
-    }
-    if (!field && (r.nfin[0] != 3 || r.nfin[1] != 3))
-      throw Fail{SG_ERR_INVALID, "unexpected threshold table at H = 0"};
-  }
+  void build_thresholds(double b, ThrRec &r) const { build_threshold_rec(b, H, r); }
 
   void reset_counters() {
     JobState js;
@@ -923,13 +927,14 @@ struct sg_handle {
     return s;
   }
 
-  // Instantiates every graph that run(n_sweeps) from the current sweep would
-  // replay, and runs nothing: a caller that times run() afterwards times no
-  // capture or instantiation.
-  void prepare(int64_t n_sweeps) {
+  // Instantiates every graph that a call run(n_sweeps) starting at sweep
+  // `from` (the current sweep when negative) would replay, and runs nothing:
+  // a caller that times that call afterwards times no capture or
+  // instantiation.
+  void prepare(int64_t from, int64_t n_sweeps) {
     if (!initialized) throw Fail{SG_ERR_STATE, "lattice not initialised"};
     if (n_sweeps < 0) throw Fail{SG_ERR_INVALID, "negative sweep count"};
-    int64_t n = sweep, remaining = n_sweeps;
+    int64_t n = from < 0 ? sweep : from, remaining = n_sweeps;
     while (remaining > 0) {
       const StepPlan s = plan_step(n, remaining);
       if (s.graphed) graph_for(s.par, (int)s.seg, s.emode, s.count);

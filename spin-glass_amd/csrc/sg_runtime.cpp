@@ -178,8 +178,8 @@ sg_status sg_run(sg_handle *h, int64_t n_sweeps, sg_obs_sink sink, void *user) {
   return guarded(h, [&] { h->run(n_sweeps, sink, user); });
 }
 
-sg_status sg_prepare(sg_handle *h, int64_t n_sweeps) {
-  return guarded(h, [&] { h->prepare(n_sweeps); });
+sg_status sg_prepare(sg_handle *h, int64_t from_sweep, int64_t n_sweeps) {
+  return guarded(h, [&] { h->prepare(from_sweep, n_sweeps); });
 }
 
 int32_t sg_graph_count(const sg_handle *h) { return h ? (int32_t)h->graphs.size() : -1; }

@@ -133,7 +133,7 @@ def load_library(path: str = LIB_PATH):
     L.sg_serve.argtypes = [_vp, ctypes.POINTER(_Params), _i64p, _dp, ctypes.c_int32, _i64p, _dp]
     L.sg_tune_count.argtypes = []
     L.sg_tune_count.restype = ctypes.c_int64
-    L.sg_prepare.argtypes = [_vp, ctypes.c_int64]
+    L.sg_prepare.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64]
     L.sg_graph_count.argtypes = [_vp]
     L.sg_probe_valu.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp]
     L.sg_probe_last_error.restype = ctypes.c_char_p
@@ -228,10 +228,11 @@ class Sim:
         cb = SINK(_cb) if (self.keep_records or sink is not None) else SINK()
         self._c(load_library().sg_run(self._h, int(n_sweeps), cb, None))
 
-    def prepare(self, n_sweeps):
-        """Instantiate every captured launch sequence that run(n_sweeps) would
-        replay from the current sweep, running nothing."""
-        self._c(load_library().sg_prepare(self._h, int(n_sweeps)))
+    def prepare(self, n_sweeps, from_sweep=-1):
+        """Instantiate every captured launch sequence that a call run(n_sweeps)
+        starting at from_sweep (default: the current sweep) would replay,
+        running nothing."""
+        self._c(load_library().sg_prepare(self._h, int(from_sweep), int(n_sweeps)))
 
     @property
     def graph_count(self):

@@ -235,6 +235,25 @@ def test_batched_form_is_bit_identical():
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("warm,steps", [(5, 20), (64, 64), (3, 7), (61, 4)])
+def test_prepared_runs_instantiate_nothing_and_match_oracle(warm, steps):
+    """sg_prepare builds every graph two split calls replay (measurement every
+    1000 sweeps, PT every 16, so the replay lengths vary near each call's end
+    and each measurement); the runs then add no graph and give the oracle's
+    bits."""
+    betas = geometric_betas(0.4, 2.0, 6)
+    sim, orc = _pair(8, 64, 2, betas, pt_every=16, meas_every=1000, form=sg.SG_FORM_BATCHED)
+    sim.prepare(warm * 16)
+    sim.prepare(steps * 16, from_sweep=warm * 16)
+    n = sim.graph_count
+    assert n > 0
+    sim.run(warm * 16)
+    sim.run(steps * 16)
+    assert sim.graph_count == n
+    orc.run((warm + steps) * 16)
+    _assert_same(sim, orc)
+
+
 def test_reference_ladder_l16_matches_oracle():
     sim, orc = _pair(16, 64, 2, sg.REF_BETAS, pt_every=16, meas_every=32)
     sim.run(40)
