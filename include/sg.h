@@ -182,6 +182,36 @@ sg_status sg_tune_kernel(int32_t device, int32_t L, int32_t n_lattices, int32_t 
 sg_status sg_tune_shapes(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
                          int32_t *kernel, double ms[4]);
 
+/* No counterpart in the reference (it has no entry point of this kind): for
+ * callers that cannot allocate device memory — code under HIP graph capture,
+ * serving loops and harnesses that forbid hipMalloc.  The sweeps with PT of
+ * sg_run (same parameters, same bits; measurements are not taken, so
+ * p->meas_every must be 0) over lattices the caller holds in device memory,
+ * allocating nothing: the working buffers (coupling site records,
+ * descriptors, thresholds, task lists, sweep counter) live in one static
+ * device array of SG_STATIC_SCRATCH_BYTES per device, held until the process
+ * ends.  Sizes whose buffers exceed it are rejected (SG_ERR_INVALID); that is
+ * G * 6 * L^3 * 8 bytes of site records plus about 100 bytes per lattice (L =
+ * 16: about 110 groups of 64 samples at 24 temperatures and 2 replicas).
+ *   spins       in/out, device, [group][replica][temp][N] uint64 in the
+ *               kernels' order: colour c = (x+y+z)&1 at c*N/2, then
+ *               (x>>1) + (L/2)(y + L z)  (sg_get_spins gives natural order)
+ *   couplings   in, device, [group][3][N] natural order (as sg_get_couplings)
+ *   sweep0      the sweep counter at entry (Philox counters and PT parity)
+ *   energies    out, device, [group][replica][temp][128] int32: U in 0..63,
+ *               K in 64..127 of the final states (as sg_get_energies)
+ *   swaps_accepted out, device, [T-1] uint64: accepted swaps of this call
+ *               (may be NULL when n_temps == 1)
+ *   stream      hipStream_t to queue the work on (NULL: the null stream);
+ *               the call returns once the work is queued.
+ * p->coupling_seed, form and kernel are not used; p->device selects the
+ * device (made current).  Calls that overlap on one device race: they share
+ * its array.  Errors: sg_last_error(NULL). */
+#define SG_STATIC_SCRATCH_BYTES (32u << 20)
+sg_status sg_run_static(const sg_params *p, uint64_t *spins, const uint64_t *couplings,
+                        int64_t sweep0, int64_t n_sweeps, int32_t *energies,
+                        uint64_t *swaps_accepted, void *stream);
+
 /* tuner measurements run in this process so far (each sg_tune_* call for an
  * unmeasured (device, L, field, bucket), and each sg_create with
  * SG_KERNEL_AUTO for one).  The batch engine never measures: its classes take

@@ -59,7 +59,7 @@ EXPORTS = [
     "sg_job_footprint", "sg_submit", "sg_step", "sg_drain", "sg_poll", "sg_result",
     "sg_job_times", "sg_batch_stats", "sg_batch_launch_stats", "sg_serve",
     "sg_probe_valu", "sg_probe_last_error", "sg_prepare", "sg_graph_count",
-    "sg_tune_count",
+    "sg_tune_count", "sg_run_static",
 ]
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
@@ -131,6 +131,8 @@ def load_library(path: str = LIB_PATH):
     L.sg_batch_stats.argtypes = [_vp, _dp, _i64p, _i64p, _i64p]
     L.sg_batch_launch_stats.argtypes = [_vp, _i64p, _i64p, _i64p]
     L.sg_serve.argtypes = [_vp, ctypes.POINTER(_Params), _i64p, _dp, ctypes.c_int32, _i64p, _dp]
+    L.sg_run_static.argtypes = [ctypes.POINTER(_Params), _vp, _vp, ctypes.c_int64, ctypes.c_int64,
+                                _vp, _vp, _vp]
     L.sg_tune_count.argtypes = []
     L.sg_tune_count.restype = ctypes.c_int64
     L.sg_prepare.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64]
@@ -332,6 +334,36 @@ def tune_shapes(L, n_lattices, field=False, device=0):
     _check(lib.sg_tune_shapes(device, L, n_lattices, 1 if field else 0, ctypes.byref(k), ms))
     return k.value, {SG_KERNEL_RESIDENT: ms[1], SG_KERNEL_STREAMED: ms[2],
                      SG_KERNEL_RESIDENT_SINGLE: ms[3]}
+
+
+STATIC_SCRATCH_BYTES = 32 << 20   # SG_STATIC_SCRATCH_BYTES
+
+
+def checkerboard_order(L):
+    """perm with kernel_order[..., perm[s]] = natural[..., s]: the site order
+    sg_run_static's spin buffers use (colour c = (x+y+z)&1 at c*N/2, then
+    (x>>1) + (L/2)(y + L z))."""
+    N = L ** 3
+    s = np.arange(N)
+    x, y, z = s % L, (s // L) % L, s // (L * L)
+    return ((x + y + z) & 1) * (N // 2) + (x >> 1) + (L // 2) * (y + L * z)
+
+
+def run_static(spins_ptr, couplings_ptr, energies_ptr, swaps_ptr, sweep0, n_sweeps, L=16,
+               n_samples=64, n_replicas=2, betas=None, T_range=None, H=0.0, spin_seed=2,
+               pt_every=1, device=0, stream=None):
+    """sg_run_static over device buffers given by address (integers): spins
+    [group][replica][temp][N] in checkerboard_order, couplings [group][3][N],
+    energies [group][replica][temp][128] int32, swaps [T-1] uint64.  Queues
+    the work on `stream` (an address, None = the null stream) and returns."""
+    lib = load_library()
+    p = _Params()
+    keep = _fill_params(p, L, n_samples, n_replicas, betas, T_range, H, 0, spin_seed, pt_every,
+                        0, device)
+    st = lib.sg_run_static(ctypes.byref(p), spins_ptr, couplings_ptr, int(sweep0), int(n_sweeps),
+                           energies_ptr, swaps_ptr, stream)
+    del keep
+    _check(st, None)
 
 
 def tune_count():
