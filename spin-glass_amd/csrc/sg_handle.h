@@ -1032,8 +1032,13 @@ struct sg_handle {
     HIPCHK(hipMemcpy(a.data(), d_acc, sizeof(ObsAcc) * a.size(), hipMemcpyDeviceToHost));
   }
 
-  // Per temperature, time means per sample from the exact sums, then the
-  // mean over the active samples.
+  // Per temperature: exact integer totals over all measurements, samples and
+  // replicas (128-bit; Q^4 sums need them at L = 48), each turned into a mean
+  // by one division, so the result does not depend on the order in which
+  // samples are reduced; the one FP64 observable (the k_min overlap, summed
+  // per lane on the device in measurement order) is added over samples in
+  // ascending order.  analyze-all.py:38-71 averages per sample first; the two
+  // agree up to rounding because every sample has the same measurement count.
   void results(double *res) {
     std::vector<ObsAcc> acc;
     read_acc(acc);
@@ -1041,30 +1046,36 @@ struct sg_handle {
   }
 
   void results_from(const ObsAcc *acc, double *res) const {
+#pragma clang fp contract(off)
     const double Nd = N;
-    const double n = (double)n_meas;
+    const double nS = (double)n_meas * S, nRS = (double)n_meas * R * S;
     for (int t = 0; t < T; ++t) {
-      double e = 0, m = 0, aq = 0, q2 = 0, q4 = 0, c1 = 0;
-      for (int s = 0; s < S && n_meas > 0; ++s) {
+      __int128 sU = 0, sK = 0, sAbsQ = 0;
+      unsigned __int128 sQ2 = 0, sQ4 = 0;
+      double sq1 = 0.0;
+      for (int s = 0; s < S; ++s) {
         const ObsAcc &a = acc[((size_t)(s / 64) * T + t) * 64 + (s % 64)];
-        const double U = (double)a.sU / (n * R), K = (double)a.sK / (n * R);
-        const double M = 2.0 * K - Nd;
-        e += ((2.0 * U - 3.0 * Nd) - H * M) / Nd;
-        m += M / Nd;
-        aq += (double)a.sAbsQ / n / Nd;
-        q2 += (double)a.sQ2 / n / (Nd * Nd);
-        const unsigned __int128 q4s = ((unsigned __int128)a.sQ4hi << 64) | a.sQ4lo;
-        q4 += (double)q4s / n / (Nd * Nd * Nd * Nd);
-        c1 += a.sq1 / n;
+        sU += a.sU;
+        sK += a.sK;
+        sAbsQ += a.sAbsQ;
+        sQ2 += a.sQ2;
+        sQ4 += ((unsigned __int128)a.sQ4hi << 64) | a.sQ4lo;
+        sq1 += a.sq1;
       }
-      const int64_t nm = n_meas;
       double *r = res + (size_t)t * SG_NRES;
-      const double ns = S;
-      e /= ns; m /= ns; aq /= ns; q2 /= ns; q4 /= ns; c1 /= ns;
+      for (int k = 0; k < SG_NRES; ++k) r[k] = 0.0;
       r[SG_RES_BETA] = beta[t];
-      r[SG_RES_E] = e;
-      r[SG_RES_M] = m;
-      r[SG_RES_Q] = aq;
+      r[SG_RES_NMEAS] = (double)n_meas;
+      r[SG_RES_BINDER] = r[SG_RES_XI_L] = NAN;
+      if (n_meas == 0) continue;
+      const double U = (double)sU / nRS, K = (double)sK / nRS;
+      const double M = 2.0 * K - Nd;
+      const double q2 = (double)sQ2 / nS / (Nd * Nd);
+      const double q4 = (double)sQ4 / nS / (Nd * Nd * Nd * Nd);
+      const double c1 = sq1 / nS;
+      r[SG_RES_E] = ((2.0 * U - 3.0 * Nd) - H * M) / Nd;
+      r[SG_RES_M] = M / Nd;
+      r[SG_RES_Q] = (double)sAbsQ / nS / Nd;
       r[SG_RES_Q2] = q2;
       r[SG_RES_Q4] = q4;
       r[SG_RES_BINDER] = q2 > 0 ? 0.5 * (3.0 - q4 / (q2 * q2)) : NAN;
@@ -1073,7 +1084,6 @@ struct sg_handle {
       r[SG_RES_XI_L] = (c1 > 0 && q2 / c1 > 1.0)
                            ? std::sqrt(q2 / c1 - 1.0) / (2.0 * std::sin(M_PI / L)) / L
                            : NAN;
-      r[SG_RES_NMEAS] = (double)nm;
     }
   }
 

@@ -12,6 +12,7 @@ point raises.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -548,12 +549,19 @@ def obs_records(rec: np.ndarray, L: int, R: int, H: float, n_samples: int) -> np
     if R >= 2:
         W = r[..., 2 * R]
         out["q_0"] = (N - 2 * W) / N
-        c = np.arange(L)
-        cw, sw = np.cos(2 * np.pi * c / L), np.sin(2 * np.pi * c / L)
+        # sum_c cos/sin(2 pi c / L) W_d[c] in FP64, c in order (the csg
+        # writer's arithmetic, so that both give the same float32 bits)
+        cw = [math.cos(2.0 * math.pi * c / L) for c in range(L)]
+        sw = [math.sin(2.0 * math.pi * c / L) for c in range(L)]
         for d in range(3):
             Wd = r[..., 2 * R + 1 + d * L: 2 * R + 1 + (d + 1) * L]
-            out["q_1_r"][..., d] = -2.0 / N * (Wd @ cw)
-            out["q_1_i"][..., d] = -2.0 / N * (Wd @ sw)
+            re = np.zeros(Wd.shape[:-1])
+            im = np.zeros(Wd.shape[:-1])
+            for c in range(L):
+                re = re + cw[c] * Wd[..., c]
+                im = im + sw[c] * Wd[..., c]
+            out["q_1_r"][..., d] = -2.0 / N * re
+            out["q_1_i"][..., d] = -2.0 / N * im
     return out
 
 

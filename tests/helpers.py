@@ -16,51 +16,73 @@ def geometric_betas(tmin, tmax, n):
 
 def results_from_records(records, L, R, H, n_samples, betas):
     """records: list of int32 arrays [group][temp][lane][field].
-    Returns [T][11] with the columns of sg.RES_COLUMNS."""
+    Returns [T][11] with the columns of sg.RES_COLUMNS.
+
+    Integer observables are totalled exactly (Python integers) over
+    measurements, samples and replicas and divided once; the k_min overlap is
+    formed per lane and measurement as an FP64 sum over planes c = 0..L-1 in
+    order (elementwise, no fused multiply-add), totalled per lane over the
+    measurements in order and over samples in ascending order.  That is the
+    arithmetic the product specifies, so the comparison can be exact."""
     N = L ** 3
+    Nd = float(N)
     T = len(betas)
-    e_s = np.zeros((n_samples, T)); m_s = np.zeros((n_samples, T))
-    aq_s = np.zeros((n_samples, T)); q1_s = np.zeros((n_samples, T))
-    Q2_s = [[0] * T for _ in range(n_samples)]
-    Q4_s = [[0] * T for _ in range(n_samples)]
-    cw = np.array([math.cos(2.0 * math.pi * c / L) for c in range(L)])
-    sw = np.array([math.sin(2.0 * math.pi * c / L) for c in range(L)])
+    cw = [math.cos(2.0 * math.pi * c / L) for c in range(L)]
+    sw = [math.sin(2.0 * math.pi * c / L) for c in range(L)]
     nm = len(records)
+    sU = [0] * T; sK = [0] * T; sAQ = [0] * T; sQ2 = [0] * T; sQ4 = [0] * T
+    sq1 = np.zeros((n_samples, T))
     for rec in records:
         G = rec.shape[0]
         r = rec.transpose(0, 2, 1, 3).reshape(G * 64, T, -1)[:n_samples].astype(np.int64)
-        for s in range(n_samples):
+        for t in range(T):
+            sU[t] += int(r[:, t, :R].sum())
+            sK[t] += int(r[:, t, R:2 * R].sum())
+        if R >= 2:
+            Q = N - 2 * r[:, :, 2 * R]
             for t in range(T):
-                f = r[s, t]
-                se = sm = 0.0
-                for rho in range(R):
-                    M = float(2 * f[R + rho] - N)
-                    se += (float(2 * f[rho] - 3 * N) - H * M) / N
-                    sm += M / N
-                e_s[s, t] += se / R
-                m_s[s, t] += sm / R
-                if R >= 2:
-                    Q = N - 2 * int(f[2 * R])
-                    Q2_s[s][t] += Q * Q
-                    Q4_s[s][t] += Q ** 4
-                    aq_s[s, t] += abs(Q) / N
-                    q1 = 0.0
-                    for d in range(3):
-                        W = f[2 * R + 1 + d * L: 2 * R + 1 + (d + 1) * L].astype(np.float64)
-                        re = -2.0 / N * float(cw @ W)
-                        im = -2.0 / N * float(sw @ W)
-                        q1 += re * re + im * im
-                    q1_s[s, t] += q1 / 3.0
+                q = [int(v) for v in Q[:, t]]
+                sAQ[t] += sum(abs(v) for v in q)
+                sQ2[t] += sum(v * v for v in q)
+                sQ4[t] += sum(v ** 4 for v in q)
+            q1 = np.zeros((n_samples, T))
+            for d in range(3):
+                W = r[:, :, 2 * R + 1 + d * L: 2 * R + 1 + (d + 1) * L].astype(np.float64)
+                re = np.zeros((n_samples, T))
+                im = np.zeros((n_samples, T))
+                for c in range(L):
+                    re = re + cw[c] * W[:, :, c]
+                    im = im + sw[c] * W[:, :, c]
+                re = re * (-2.0 / N)
+                im = im * (-2.0 / N)
+                q1 = q1 + (re * re + im * im)
+            sq1 = sq1 + q1 / 3.0
     out = np.zeros((T, 11))
+    nS = float(nm * n_samples)
+    nRS = float(nm * R * n_samples)
     for t in range(T):
-        e = np.mean(e_s[:, t] / nm)
-        m = np.mean(m_s[:, t] / nm)
-        aq = np.mean(aq_s[:, t] / nm)
-        q2 = np.mean([Q2_s[s][t] / nm / N ** 2 for s in range(n_samples)])
-        q4 = np.mean([Q4_s[s][t] / nm / N ** 4 for s in range(n_samples)])
-        c1 = np.mean(q1_s[:, t] / nm)
-        out[t] = [betas[t], e, m, aq, q2, q4,
-                  0.5 * (3 - q4 / q2 ** 2) if q2 > 0 else np.nan, q2, c1,
-                  math.sqrt(q2 / c1 - 1) / (2 * math.sin(math.pi / L)) / L
-                  if c1 > 0 and q2 / c1 > 1 else np.nan, nm]
+        out[t, 0] = betas[t]
+        out[t, 10] = nm
+        out[t, 6] = out[t, 9] = np.nan
+        if nm == 0:
+            continue
+        U = float(sU[t]) / nRS
+        K = float(sK[t]) / nRS
+        M = 2.0 * K - Nd
+        q2 = float(sQ2[t]) / nS / (Nd * Nd)
+        q4 = float(sQ4[t]) / nS / (Nd * Nd * Nd * Nd)
+        c = 0.0
+        for s_ in range(n_samples):
+            c += float(sq1[s_, t])
+        c1 = c / nS
+        out[t, 1] = ((2.0 * U - 3.0 * Nd) - H * M) / Nd
+        out[t, 2] = M / Nd
+        out[t, 3] = float(sAQ[t]) / nS / Nd
+        out[t, 4] = q2
+        out[t, 5] = q4
+        out[t, 6] = 0.5 * (3.0 - q4 / (q2 * q2)) if q2 > 0 else np.nan
+        out[t, 7] = q2
+        out[t, 8] = c1
+        out[t, 9] = (math.sqrt(q2 / c1 - 1.0) / (2.0 * math.sin(math.pi / L)) / L
+                     if c1 > 0 and q2 / c1 > 1.0 else np.nan)
     return out

@@ -156,7 +156,7 @@ def test_engine_job_matches_oracle():
     orc.init()
     orc.run(24)
     ref = results_from_records([orc.meas(i) for i in range(orc.n_meas())], 8, 2, 0.0, 80, betas)
-    np.testing.assert_allclose(res, ref, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(res, ref)
     np.testing.assert_array_equal(acc, orc.swap_counts()[0])
     b.close()
 
@@ -255,6 +255,72 @@ def test_mixed_benchmark_batch_at_full_size_matches_oracle():
         orc.run(sweeps)
         ref = results_from_records([orc.meas(k) for k in range(orc.n_meas())], j.L, 2, 0.0, 64,
                                    betas)
-        np.testing.assert_allclose(res, ref, rtol=1e-12, atol=1e-13, err_msg=f"L={j.L}")
+        np.testing.assert_array_equal(res, ref, err_msg=f"L={j.L}")
         np.testing.assert_array_equal(acc, orc.swap_counts()[0], err_msg=f"L={j.L}")
     b.close()
+
+
+def _serving_stream(n, rate, sweeps_lo, sweeps_hi, seed):
+    """A job stream shaped like the serving workload (Poisson arrivals at
+    `rate`, L ~ U{8,12,16,20,24}, 16..32 temperatures geometric in [0.2, 2],
+    its own coupling and spin seeds, sweeps log-uniform in [lo, hi] rounded to
+    the PT cadence, a measurement every 1024 sweeps)."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    arrivals -= arrivals[0]
+    jobs = []
+    for i in range(n):
+        L = int(rng.choice((8, 12, 16, 20, 24)))
+        sw = float(np.exp(rng.uniform(np.log(sweeps_lo), np.log(sweeps_hi))))
+        sw = max(PT, int(round(sw / PT)) * PT)
+        T = int(rng.integers(16, 33))
+        jobs.append(sg.Job(L=L, n_sweeps=sw, n_samples=64, T_range=(0.2, 2.0, T),
+                           coupling_seed=0x5EED2000 + i, spin_seed=0x5EED3000 + i,
+                           meas_every=1024))
+    return jobs, arrivals
+
+
+@pytest.mark.gpu
+def test_serving_workload_jobs_equal_their_alone_runs():
+    """The serving configuration, scaled to test time: 24 jobs arriving as a
+    Poisson stream at 200/s (seed 0x5EED0005), L from {8..24}, 16-32
+    temperatures, 1e3-4e3 sweeps, own seeds, served by sg_serve with continuous
+    batching.  Every job's FP64 results and accepted swap counts are
+    bit-identical to the same parameters run alone through sg_run; the
+    smallest-L job also matches the oracle; the loop runs the tuner zero
+    times."""
+    jobs, arrivals = _serving_stream(24, 200.0, 1e3, 4e3, 0x5EED0005)
+    assert len({j.L for j in jobs}) >= 4
+    b = sg.Batch(pt_every=PT, slice_pt_steps=2, arena_bytes=4 << 30)
+    tunes = sg.tune_count()
+    ids, wall = b.serve(jobs, arrivals)
+    assert sg.tune_count() == tunes
+    lat = [b.times(i)[2] - b.times(i)[0] for i in ids]
+    served = [b.result(i) for i in ids]     # releases each job
+    print(f"serve: {len(jobs)} jobs in {wall:.3f} s, p50 {np.percentile(lat, 50):.4f} s, "
+          f"p99 {np.percentile(lat, 99):.4f} s", flush=True)
+    b.close()
+    for j, (res, acc) in zip(jobs, served):
+        s = sg.Sim(L=j.L, n_samples=64, n_replicas=2, T_range=j.T_range,
+                   coupling_seed=j.coupling_seed, spin_seed=j.spin_seed, pt_every=PT,
+                   meas_every=j.meas_every)
+        s.keep_records = False
+        s.init_lattice()
+        s.run(j.n_sweeps)
+        np.testing.assert_array_equal(res, s.results(), err_msg=f"L={j.L} sweeps={j.n_sweeps}")
+        np.testing.assert_array_equal(acc, s.swap_counts()[0], err_msg=f"L={j.L}")
+        s.close()
+    k = int(np.argmin([j.L * 100000 + j.n_sweeps for j in jobs]))
+    j = jobs[k]
+    probe = sg.Sim(L=j.L, n_samples=64, n_replicas=2, T_range=j.T_range)
+    betas = probe.betas
+    probe.close()
+    oracle.set_threads(8)
+    orc = oracle.OracleSim(j.L, 1, 2, betas, coupling_seed=j.coupling_seed,
+                           spin_seed=j.spin_seed, pt_every=PT, meas_every=j.meas_every)
+    orc.init()
+    orc.run(j.n_sweeps)
+    ref = results_from_records([orc.meas(i) for i in range(orc.n_meas())], j.L, 2, 0.0, 64,
+                               betas)
+    np.testing.assert_array_equal(served[k][0], ref)
+    np.testing.assert_array_equal(served[k][1], orc.swap_counts()[0])

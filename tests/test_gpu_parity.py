@@ -1,7 +1,9 @@
 """Bit-exact parity of libsg (HIP, through the C-ABI) with the CPU oracle.
 
 Every spin word, per-lane energy, measurement record and swap count must be
-identical; the FP64 results must agree within 1e-12 relative."""
+identical, and so must the FP64 results (exact integer totals and one
+fixed-order FP64 sum, restated by tests/helpers.py), well inside the 1e-12
+relative tolerance the results are specified to."""
 import numpy as np
 import pytest
 
@@ -61,7 +63,7 @@ def test_sweeps_pt_measurements_match_oracle(L):
         assert sweep == 3 * (i + 1)
         np.testing.assert_array_equal(rec, orc.meas(i))
     ref = results_from_records([orc.meas(i) for i in range(orc.n_meas())], L, 2, 0.0, 80, betas)
-    np.testing.assert_allclose(sim.results(), ref, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(sim.results(), ref)
 
 
 @pytest.mark.parametrize("L", [18, 20, 22, 24, 26, 28, 32, 48])
@@ -83,6 +85,22 @@ def test_large_lattice_sweeps_match_oracle(L):
     assert len(sim.records) == orc.n_meas() == 2
     for i, (_, rec) in enumerate(sim.records):
         np.testing.assert_array_equal(rec, orc.meas(i))
+
+
+@pytest.mark.parametrize("kernel", [sg.SG_KERNEL_RESIDENT, sg.SG_KERNEL_STREAMED])
+@pytest.mark.parametrize("H", [0.0, 0.5])
+@pytest.mark.parametrize("L", [30, 36, 46])
+def test_other_large_sizes_match_oracle(L, H, kernel):
+    """Sizes the benchmarks do not use: k_big (resident, in L2) and k_phase
+    (streamed colour launches) at L = 30 and 46 (L = 2 mod 4: a colour row
+    holds an odd number of words) and 36, at H = 0 and H != 0."""
+    betas = geometric_betas(0.3, 2.0, 3)
+    sim, orc = _pair(L, 64, 2, betas, H=H, pt_every=2, meas_every=3, kernel=kernel)
+    sim.run(4)
+    orc.run(4)
+    _assert_same(sim, orc)
+    assert len(sim.records) == orc.n_meas() == 1
+    np.testing.assert_array_equal(sim.records[0][1], orc.meas(0))
 
 
 @pytest.mark.parametrize("case", [
@@ -276,7 +294,7 @@ def test_benchmark_workload_at_full_size_matches_oracle():
     assert len(sim.records) == orc.n_meas() == 1
     np.testing.assert_array_equal(sim.records[0][1], orc.meas(0))
     ref = results_from_records([orc.meas(0)], 16, 2, 0.0, 2048, betas)
-    np.testing.assert_allclose(sim.results(), ref, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(sim.results(), ref)
 
 
 @pytest.mark.parametrize("L", [32, 48])
@@ -293,7 +311,7 @@ def test_large_lattice_benchmark_at_full_size_matches_oracle(L):
     assert len(sim.records) == orc.n_meas() == 1
     np.testing.assert_array_equal(sim.records[0][1], orc.meas(0))
     ref = results_from_records([orc.meas(0)], L, 2, 0.0, 64, betas)
-    np.testing.assert_allclose(sim.results(), ref, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(sim.results(), ref)
 
 
 def test_checkpoint_resume_is_bit_exact():
@@ -364,3 +382,69 @@ def test_cli_resume_equals_straight_run(tmp_path):
     assert rows.shape == (6, 9)
     assert np.all(rows[:, 5] < 0)            # e < 0 at every temperature
     assert np.all(np.diff(rows[:, 5]) > 0)   # and rises with T
+
+
+CLI_FLAGS = ["--L", "8", "--samples", "96", "--meas-every", "64", "--temps", "6",
+             "--tmin", "0.5", "--tmax", "3.0", "--pt-every", "4"]
+
+
+def _csg():
+    import os
+    return os.path.join(os.path.dirname(sg.__file__), "bin", "csg")
+
+
+def test_cli_obs_bin_equals_obs_records_of_the_oracle(tmp_path):
+    """The csg writer's float32 obs_s records (eamsim.cuh:55-62) are byte-equal
+    to sg.obs_records applied to the oracle's integer records of the same
+    run (H from std::stof, the handle's own ladder and the CLI's default
+    seeds)."""
+    import subprocess
+    d = tmp_path / "08"
+    d.mkdir()
+    subprocess.run([_csg(), str(d), "0.25", "--sweeps", "256", *CLI_FLAGS], check=True,
+                   cwd=tmp_path, capture_output=True)
+    got = (d / "0.2500-obs.bin").read_bytes()
+    probe = sg.Sim(L=8, n_samples=96, n_replicas=2, T_range=(0.5, 3.0, 6))
+    betas = probe.betas
+    probe.close()
+    orc = oracle.OracleSim(8, 2, 2, betas, H=0.25, coupling_seed=CSEED, spin_seed=SSEED,
+                           pt_every=4, meas_every=64)
+    orc.init()
+    orc.run(256)
+    assert orc.n_meas() == 4
+    want = b"".join(sg.obs_records(orc.meas(i), 8, 2, 0.25, 96).tobytes()
+                    for i in range(orc.n_meas()))
+    assert len(got) == 4 * 96 * 6 * 44
+    assert got == want
+
+
+def test_cli_interrupted_run_resumes_to_the_uninterrupted_obs_bin(tmp_path):
+    """A run stopped after 5 records (checkpoints every 2 measurements, so the
+    last checkpoint holds 4) leaves 5 records in obs.bin; the next invocation
+    cuts obs.bin back to the checkpoint's length, finishes the interrupted
+    run, and leaves exactly the obs.bin of an uninterrupted run; a further
+    invocation then appends the same records to both."""
+    import subprocess
+    a, b = tmp_path / "a" / "08", tmp_path / "b" / "08"
+    a.mkdir(parents=True)
+    b.mkdir(parents=True)
+    run = ["--sweeps", "512", "--checkpoint-every", "2", *CLI_FLAGS]
+    subprocess.run([_csg(), str(a), "0.25", *run], check=True, cwd=tmp_path,
+                   capture_output=True)
+    r = subprocess.run([_csg(), str(b), "0.25", *run, "--stop-after-records", "5"],
+                       cwd=tmp_path, capture_output=True)
+    assert r.returncode != 0
+    rec = 96 * 6 * 44
+    assert len((b / "0.2500-obs.bin").read_bytes()) == 5 * rec
+    assert list(tmp_path.glob("*.log"))          # a failed run keeps its log
+    r = subprocess.run([_csg(), str(b), "0.25", *run], check=True, cwd=tmp_path,
+                       capture_output=True)
+    assert b"resuming an interrupted run at sweep 256 of 512" in r.stdout
+    ra = (a / "0.2500-obs.bin").read_bytes()
+    assert len(ra) == 8 * rec
+    assert (b / "0.2500-obs.bin").read_bytes() == ra
+    for d in (a, b):
+        subprocess.run([_csg(), str(d), "0.25", *run], check=True, cwd=tmp_path,
+                       capture_output=True)
+    assert (a / "0.2500-obs.bin").read_bytes() == (b / "0.2500-obs.bin").read_bytes()
+    assert len((a / "0.2500-obs.bin").read_bytes()) == 16 * rec

@@ -1,8 +1,9 @@
 """sg_run_static, the entry point that allocates nothing, against sg_run.
 
-Two calls in a row on one device (the second smaller, at H != 0 and from a
-non-zero sweep counter), each checked bit for bit against a handle running
-the same parameters through sg_run: spins, per-lane U and K, accepted swap
+Calls in a row on one device (the second smaller, at H != 0 and from a
+non-zero sweep counter; a third at L = 22, where sg_run holds one lattice per
+work-group and sg_run_static two), each checked bit for bit against a handle
+running the same parameters through sg_run: spins, per-lane U and K, accepted swap
 counts.  The static array keeps the first call's contents; the second must
 not depend on them.  Device buffers come from torch (plumbing only)."""
 import numpy as np
@@ -16,6 +17,8 @@ CASES = [
     # L, samples, T, R, H, pt_every, sweep0, n_sweeps
     dict(L=16, n_samples=128, T=9, R=2, H=0.0, pt_every=4, sweep0=0, n_sweeps=42),
     dict(L=8, n_samples=40, T=5, R=3, H=0.5, pt_every=3, sweep0=7, n_sweeps=20),
+    # L = 22: sg_run_static's two-lattice kernel against sg_run's one-lattice one
+    dict(L=22, n_samples=64, T=3, R=2, H=0.25, pt_every=2, sweep0=0, n_sweeps=6),
 ]
 
 
