@@ -76,7 +76,7 @@ def make(samples, form):
 
 
 SHAPE_NAMES = {sg.SG_KERNEL_RESIDENT: "lattice pairs", sg.SG_KERNEL_RESIDENT_SINGLE: "single lattices",
-               sg.SG_KERNEL_STREAMED: "streamed chunks"}
+               sg.SG_KERNEL_STREAMED: "streamed chunks", sg.SG_KERNEL_SPLIT: "split lattices"}
 
 
 CLOCK_WARM_STEPS = 160  # about 27 ms of the timed workload's own launches

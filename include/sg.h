@@ -73,9 +73,14 @@ enum {
   SG_KERNEL_AUTO = 0,     /* chosen per L (and lattice count) from measured rates */
   SG_KERNEL_RESIDENT = 1, /* lattice pair resident in one work-group for a PT interval */
   SG_KERNEL_STREAMED = 2, /* one launch per colour, lattices streamed in chunks (L >= 12) */
-  SG_KERNEL_RESIDENT_SINGLE = 3 /* one lattice per work-group (twice the work-groups of
+  SG_KERNEL_RESIDENT_SINGLE = 3, /* one lattice per work-group (twice the work-groups of
                                    RESIDENT), PT in a launch of its own; the same as
                                    RESIDENT at L = 22..26, whose kernel holds one lattice */
+  SG_KERNEL_SPLIT = 4     /* L = 32 (<= 256 lattices per launch): a lattice held on
+                             chip by P = 1, 2 or 4 work-groups for a PT interval,
+                             exchanging boundary planes (all P work-groups of a
+                             lattice run at once); PT in a launch of its own.  Not
+                             used by the batch engine */
 };
 
 /* Per-measurement callback: rec holds n_groups*n_temps*64 records of nfield
@@ -177,10 +182,10 @@ sg_status sg_get_kernel_stats(sg_handle *h, int32_t kind, int64_t *launches,
 sg_status sg_tune_kernel(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
                          int32_t *kernel, double *resident_ms, double *streamed_ms);
 /* the same decision with the time of every shape: ms[k] for k =
- * SG_KERNEL_RESIDENT, _STREAMED, _RESIDENT_SINGLE (ms[0] unused; 0 where the
- * L has no such shape) */
+ * SG_KERNEL_RESIDENT, _STREAMED, _RESIDENT_SINGLE, _SPLIT (ms[0] unused; 0
+ * where the L has no such shape) */
 sg_status sg_tune_shapes(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
-                         int32_t *kernel, double ms[4]);
+                         int32_t *kernel, double ms[5]);
 
 /* No counterpart in the reference (it has no entry point of this kind): for
  * callers that cannot allocate device memory — code under HIP graph capture,

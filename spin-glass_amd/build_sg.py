@@ -26,7 +26,7 @@ FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-ffp-contract=
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}"]
 
-LIB_SOURCES = ["sg_kernels_resident.hip", "sg_kernels.hip", "sg_runtime.cpp", "sg_engine.cpp",
+LIB_SOURCES = ["sg_kernels_resident.hip", "sg_kernels_split.hip", "sg_kernels.hip", "sg_runtime.cpp", "sg_engine.cpp",
                "sg_probe.hip", "sg_static.hip"]
 # The resident sweep kernels take LLVM's max-ILP machine scheduler (faster for
 # them; slower for the streamed kernels, which keep the default).

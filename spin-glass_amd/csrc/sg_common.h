@@ -154,6 +154,13 @@ struct JobPtrs {
   int32_t *energy;
   unsigned long long *counters;
   uint64_t *spins2;    // slab sweeps: the second spin buffer (same layout), or null
+  // split sweeps (a lattice held by P work-groups): boundary-plane exchange
+  // buffer [lattice][P][parity][half][side][L^2/2], the phase flag of each
+  // (lattice, slab) and an error word set by a wait that timed out; null
+  // where the job has no split shape
+  uint64_t *xbuf;
+  uint32_t *xflag;
+  uint32_t *xerr;
 };
 
 // One work-group task of the sweep kernel: lattice descriptor indices (the

@@ -224,7 +224,7 @@ struct sg_batch {
     j->p.device = bp.device;
     j->p.form = SG_FORM_BATCHED;
     if (in.kernel < SG_KERNEL_AUTO || in.kernel > SG_KERNEL_RESIDENT_SINGLE)
-      throw Fail{SG_ERR_INVALID, "unknown kernel"};
+      throw Fail{SG_ERR_INVALID, "unknown kernel (the engine takes no split shape)"};
     if (in.kernel == SG_KERNEL_STREAMED && !phase_kernel_supported(in.L))
       throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
     if (in.betas) {

@@ -287,6 +287,15 @@ struct sg_handle {
   // group (task_off[p][g] .. task_off[p][g+1])
   TaskDesc *d_tasks[2] = {nullptr, nullptr};
   std::vector<int> task_off[2];
+  // split sweeps (SG_KERNEL_SPLIT, L = 32, 40, 48): a lattice held by P
+  // work-groups for a PT interval (P from the lattices per launch); one task
+  // per lattice; boundary-plane exchange buffer, phase flags, error word and
+  // the launch's ticket counters
+  bool can_split = false, split_shape = false;
+  int split_P = 0;
+  TaskDesc *d_split = nullptr;
+  uint64_t *d_xbuf = nullptr;
+  uint32_t *d_xflag = nullptr, *d_xerr = nullptr, *d_ticket = nullptr;
   // one lattice per work-group with a two-lattice kernel
   // (SG_KERNEL_RESIDENT_SINGLE): every lattice its own task, the same list
   // for both parities, PT in its own launch; ordered by group (single_off)
@@ -369,6 +378,7 @@ struct sg_handle {
     dev_free(d_acc); dev_free(d_cs); dev_free(d_tasks[0]); dev_free(d_tasks[1]); dev_free(d_single);
     dev_free(d_self); dev_free(d_Js); dev_free(d_chunks); dev_free(d_ptc[0]); dev_free(d_ptc[1]);
     dev_free(d_spins2); dev_free(d_slabs);
+    dev_free(d_split); dev_free(d_xbuf); dev_free(d_xflag); dev_free(d_xerr); dev_free(d_ticket);
     for (auto &g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();
     for (int i = 0; i < kRing; ++i) {
@@ -425,8 +435,13 @@ struct sg_handle {
 
     can_stream = phase_kernel_supported(L);
     can_single = !single_kernel_supported(L);
+    // split sweeps need the streamed kernels too (energy-only passes)
+    split_P = split_slabs(L, form == SG_FORM_PLAIN ? R * T : (int)((size_t)G * R * T));
+    can_split = split_P > 0 && can_stream && !arena;
     if (p.kernel == SG_KERNEL_STREAMED && !can_stream)
       throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
+    if (p.kernel == SG_KERNEL_SPLIT && !can_split)
+      throw Fail{SG_ERR_INVALID, "no split kernel for this L and size"};
     dev_alloc(&d_spins, n_lat() * N);
     dev_alloc(&d_J, (size_t)G * 3 * N);
     dev_alloc(&d_Js, (size_t)G * 6 * N);
@@ -582,8 +597,10 @@ struct sg_handle {
         dev_alloc(&d_spins2, n_lat() * N);
       }
     }
+    if (can_split) build_split();
     self.spins = d_spins; self.J = d_J; self.Js = d_Js;
     self.spins2 = d_spins2;
+    self.xbuf = d_xbuf; self.xflag = d_xflag; self.xerr = d_xerr;
     self.descs = d_lat; self.pairs = d_pairs;
     self.thr = d_thr; self.js = d_jobs; self.energy = d_energy; self.counters = d_cnt;
     dev_alloc(&d_self, 1);
@@ -600,10 +617,13 @@ struct sg_handle {
   // old launch sequence
   void set_mode(int k) {
     const bool s = k == SG_KERNEL_STREAMED, one = k == SG_KERNEL_RESIDENT_SINGLE && can_single;
+    const bool sp = k == SG_KERNEL_SPLIT;
     if (s && !can_stream) throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
-    if (s == streamed && one == single_shape) return;
+    if (sp && !can_split) throw Fail{SG_ERR_INVALID, "no split kernel for this L and size"};
+    if (s == streamed && one == single_shape && sp == split_shape) return;
     streamed = s;
     single_shape = one;
+    split_shape = sp;
     for (auto &g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();
   }
@@ -627,6 +647,43 @@ struct sg_handle {
     dev_alloc(&d_single, h_single.size());
     HIPCHK(hipMemcpy(d_single, h_single.data(), sizeof(TaskDesc) * h_single.size(),
                      hipMemcpyHostToDevice));
+  }
+
+  void build_split() {
+    std::vector<TaskDesc> tv(n_lat());
+    for (size_t i = 0; i < n_lat(); ++i) tv[i] = TaskDesc{{(uint32_t)i, 0xFFFFFFFFu}, kNoPair, 0};
+    dev_alloc(&d_split, tv.size());
+    HIPCHK(hipMemcpy(d_split, tv.data(), sizeof(TaskDesc) * tv.size(), hipMemcpyHostToDevice));
+    const size_t PL = (size_t)L * L / 2;
+    dev_alloc(&d_xbuf, n_lat() * split_P * 8 * PL);
+    dev_alloc(&d_xflag, n_lat() * split_P);
+    dev_alloc(&d_xerr, 1);
+    dev_alloc(&d_ticket, 2);
+    HIPCHK(hipMemset(d_xflag, 0, sizeof(uint32_t) * n_lat() * split_P));
+    HIPCHK(hipMemset(d_xerr, 0, sizeof(uint32_t)));
+    HIPCHK(hipMemset(d_ticket, 0, sizeof(uint32_t) * 2));
+  }
+
+  // the split launches' phase flags hold absolute phases 2 sweep + c + 1;
+  // wherever the sweep counter may move back (init, load, an aborted launch)
+  // they are cleared at once, in stream order before any later launch (a
+  // captured interval then finds them cleared too)
+  void clear_split_flags() {
+    if (!can_split) return;
+    HIPCHK(launch_fill_u32(d_xflag, 0u, (int)(n_lat() * split_P), st));
+  }
+
+  // the error word of the split launches: a halo wait that timed out
+  void check_split_error() {
+    if (!can_split) return;
+    uint32_t e = 0;
+    HIPCHK(hipMemcpy(&e, d_xerr, sizeof(e), hipMemcpyDeviceToHost));
+    if (e) {
+      HIPCHK(hipMemset(d_xerr, 0, sizeof(uint32_t)));
+      HIPCHK(hipMemset(d_ticket, 0, sizeof(uint32_t) * 2));
+      clear_split_flags();
+      throw Fail{SG_ERR_DEVICE, "split sweeps: a boundary-plane wait timed out (results invalid)"};
+    }
   }
 
   void build_stream_tasks() {
@@ -742,6 +799,7 @@ struct sg_handle {
     reset_counters();
     energy_valid = energy_k_valid = false;
     initialized = true;
+    clear_split_flags();
   }
 
   // natural <-> checkerboard-split order
@@ -778,7 +836,11 @@ struct sg_handle {
   // batched form, one launch per group in the plain form), then the device
   // sweep counter advance.  do_pt: the launch starts at a PT step.
   void launch_interval(int par, int do_pt, int seg, int want_energy) {
-    if (streamed) {
+    if (split_shape && seg > 0) {
+      launch_interval_split(par, do_pt, seg, want_energy);
+      return;
+    }
+    if (streamed || split_shape) {
       launch_interval_streamed(par, do_pt, seg, want_energy);
       return;
     }
@@ -809,7 +871,7 @@ struct sg_handle {
   }
 
   void launch_energy_all(int mode) {
-    if (streamed) {
+    if (streamed || split_shape) {
       launch_interval_streamed(0, 0, 0, mode);
       return;
     }
@@ -817,6 +879,25 @@ struct sg_handle {
       HIPCHK(launch_single(L, field, d_self, d_tasks[0], task_off[0][G], 0, mode, st));
     else
       HIPCHK(launch_block(L, field, d_self, d_tasks[0], task_off[0][G], 0, 0, mode, st));
+  }
+
+  // Split interval: the PT launch over the (pair, chunk) tasks, then one split
+  // launch (all lattices of the launch, P work-groups each) for the seg sweeps
+  // and the energies.
+  void launch_interval_split(int par, int do_pt, int seg, int want_energy) {
+    const bool per_group = form == SG_FORM_PLAIN;
+    const int per = R * T;
+    for (int g = 0; g < (per_group ? G : 1); ++g) {
+      const int pa = per_group ? ptc_off[par][g] : 0;
+      const int pb = per_group ? ptc_off[par][g + 1] : ptc_off[par][G];
+      if (do_pt) HIPCHK(launch_pt_chunks(d_self, d_ptc[par] + pa, pb - pa, st));
+      const int a = per_group ? g * per : 0, n = per_group ? per : (int)n_lat();
+      const double fl = (double)seg * N * 64.0 * n;
+      timed(0, fl, [&] {
+        HIPCHK(launch_split(L, field, d_self, d_split + a, n, split_P, d_ticket, seg, want_energy, st));
+      });
+    }
+    HIPCHK(launch_advance(d_self, 1, seg, st));
   }
 
   // Streamed interval: the PT launch, then one launch per colour and sweep;
@@ -977,6 +1058,7 @@ struct sg_handle {
     HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
     kernel_ms += ms;
     collect_profile();
+    if (split_shape) check_split_error();
   }
 
   // The measurement folds itself into the device sums (ObsAcc); records leave
@@ -1166,6 +1248,7 @@ struct sg_handle {
     n_meas = h.n_meas;
     reset_counters();
     energy_valid = energy_k_valid = false;
+    clear_split_flags();
   }
 };
 

@@ -64,11 +64,27 @@ bool slab_kernel_supported(int L);
 hipError_t launch_slab(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
                        int dir, int it, int mode, hipStream_t st);
 
+// ---- split sweeps: a lattice (L = 32) held on chip by P work-groups
+// for a whole PT interval, exchanging boundary planes through jt's xbuf /
+// xflag (k_split).  split_slabs: the P used for n_lattices lattices of size
+// L (0: no split kernel for this L); P > 1 needs all n_lattices * P
+// work-groups resident at once (one per CU), so callers keep n_lattices * P
+// <= 256 and never run two split launches at the same time.
+int split_slabs(int L, int n_lattices);
+// Tasks: lat[0] = lattice descriptor, one per lattice; ticket: two zeroed
+// uint32 counters kept for this task list (the launch leaves them zeroed);
+// nsweeps >= 1; want_energy as in launch_block.  PT runs before the launch.
+hipError_t launch_split(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int nlat,
+                        int P, uint32_t *ticket, int nsweeps, int want_energy, hipStream_t st);
+
 // Tasks: lat[0] = PairDesc index, lat[1] = chunk, job.
 hipError_t launch_pt_chunks(const JobPtrs *jt, const TaskDesc *tasks, int ntasks, hipStream_t st);
 // coupling site records of one group from its [3][N] couplings
 hipError_t launch_build_jsite(int L, uint64_t *Js, const uint64_t *J, uint64_t j_off,
                               hipStream_t st);
+
+// p[0..n) = v (a kernel, so that it may sit in a captured launch sequence)
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, int n, hipStream_t st);
 
 // adds nsweeps to the sweep counter of each of the njobs jobs of jt
 hipError_t launch_advance(const JobPtrs *jt, int njobs, int nsweeps, hipStream_t st);

@@ -9,6 +9,13 @@
 // 112-208 (one curand() and one LUT read per spin there).
 #include "sg_kernels.h"
 
+// This file is compiled three times: as itself (everything but the resident
+// and split sweep kernels), through sg_kernels_resident.hip
+// (SG_RESIDENT_PART) and through sg_kernels_split.hip (SG_SPLIT_PART).
+#if !defined(SG_RESIDENT_PART) && !defined(SG_SPLIT_PART)
+#define SG_MAIN_PART 1
+#endif
+
 namespace sg {
 
 __device__ __forceinline__ uint64_t maj3(uint64_t a, uint64_t b, uint64_t c) {
@@ -1682,7 +1689,7 @@ __global__ __launch_bounds__(256) void k_init_spins(uint64_t *__restrict__ spins
   spins[d.spin_off + cb_index<L>(x, y, z)] = (uint64_t)r.v[0] | ((uint64_t)r.v[1] << 32);
 }
 
-#ifndef SG_RESIDENT_PART  // non-template kernels: defined once, in sg_kernels.hip
+#ifdef SG_MAIN_PART  // non-template kernels: defined once, in sg_kernels.hip
 __global__ __launch_bounds__(256) void k_init_couplings(uint64_t *__restrict__ J, uint64_t j_off,
                                                         int N, uint32_t g, uint32_t key0,
                                                         uint32_t key1) {
@@ -1691,6 +1698,11 @@ __global__ __launch_bounds__(256) void k_init_couplings(uint64_t *__restrict__ J
   if (s >= N) return;
   const u32x4 r = philox((uint32_t)s, (uint32_t)dir, g, kTagCoup << 28, key0, key1);
   J[j_off + (uint64_t)dir * N + s] = (uint64_t)r.v[0] | ((uint64_t)r.v[1] << 32);
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 __global__ void k_advance(const JobPtrs *jt, int njobs, int nsweeps) {
@@ -2044,7 +2056,7 @@ __global__ __launch_bounds__(SlabGeo<L>::NT) void k_slab(const JobPtrs *__restri
   }
 }
 
-#ifndef SG_RESIDENT_PART  // non-template kernels: defined once, in sg_kernels.hip
+#ifdef SG_MAIN_PART  // non-template kernels: defined once, in sg_kernels.hip
 // PT step of streamed lattices: a task is (pair, chunk of kPtWords words);
 // every work-group decides the 64 lanes itself (same energies, same Philox
 // words, so the same mask) and exchanges its chunk; chunk 0 counts.
@@ -2088,6 +2100,310 @@ __global__ __launch_bounds__(256) void k_pt_chunk(const JobPtrs *__restrict__ jt
 }
 
 #endif
+
+// ------------------------------------------------------------- split kernel
+// A lattice too large for one CU's LDS held on chip by P work-groups for a
+// whole PT interval (instantiated for L = 32, P = 4).  Work-group w of a
+// lattice owns the slab pair z in [wD, wD + D) and [wD + L/2, wD + D + L/2),
+// D = L / 2P, so each Philox quad it owns has all four of its sites there.
+// Its sites of both colours stay in VGPRs: thread t owns the quads
+// qi = t + k NT (k = 0, 1) of the slab pair, one on a boundary plane of the
+// halves (zl = 0 or D - 1) and one inside.  LDS holds a copy of each colour:
+// the slab pair's own planes plus a halo plane below and above each half, so
+// an update reads five neighbours from LDS and the sixth (the other colour's
+// word of the same index, same quad) from a register.
+//
+// A phase (one colour c) updates each thread's boundary site first, writes
+// the new colour-c words into the LDS copy of colour c and the boundary
+// planes into the exchange buffer, and publishes the phase in its flag; it
+// then updates the inner sites while the planes travel, waits for its two
+// neighbours' flags and copies their planes into the halo rows of colour c.
+// Exchange data and flags go through device-coherent accesses (relaxed
+// agent-scope atomics, which pass the per-XCD L2s to the memory side); each
+// thread waits for its plane stores to complete before the barrier that
+// precedes the flag store, so the planes are visible wherever the flag is.
+// No L2 write-back or invalidate is needed (the L2s keep the coupling
+// records).  The flag holds the absolute phase 2 (sweep + it) + c + 1, so
+// flags only grow while the sweep counter does (the host clears them
+// whenever the counter is set back); the buffer is double-buffered by phase
+// parity, and a work-group is never more than one phase ahead of a
+// neighbour.
+//
+// Work-groups take their (lattice, slab pair) from a ticket counter, so
+// the slab pairs of one lattice go to work-groups dispatched together and a
+// lattice never waits on a work-group that cannot start; the last
+// work-group to finish resets the counter for the next launch.  A wait gives
+// up after kSplitSpin polls (far beyond any normal wait) and sets xerr, which
+// the host turns into an error.
+//
+// PT runs before the launch (k_pt_chunk); the launch sweeps nsweeps >= 1
+// times and, with want_energy, counts U (colour-0 ends, in registers) and K
+// per work-group; with P > 1 the slab pair w = 0 zeroes the lattice's energy
+// slots before its first flag and every work-group adds its counts at the
+// end, after its last exchange, which orders the adds after the zeroing.
+constexpr uint32_t kSplitSpin = 1u << 19;
+
+template <int L, int P>
+struct SplitGeo {
+  static constexpr int N = L * L * L;
+  static constexpr int NH = N / 2;
+  static constexpr int LH = L / 2;
+  static constexpr int PL = LH * L;             // words per plane of one colour
+  static constexpr int D = L / (2 * P);         // planes per half slab
+  static constexpr int QW = D * PL;             // quads per slab pair
+  static constexpr int NT = QW / 2;             // two quads per thread
+  static constexpr int ROWS = D + 2;            // halo, D own planes, halo
+  static constexpr int XW = 8 * PL;             // exchange words per (lattice, slab pair)
+  static constexpr int PU = ceil_log2(12 * 2 + 1);
+  static constexpr int PK = ceil_log2(4 * 2 + 1);
+  static_assert(L % (2 * P) == 0 && D == 4 && NT == 2 * PL && NT <= 1024,
+                "threads t < PL own planes 0 and 2, the others 1 and 3");
+  static_assert(2 * 2 * ROWS * PL * 8 <= 150 * 1024, "LDS copies of both colours");
+};
+
+// A value the compiler must recompute where it is used: keeps per-quad
+// addresses out of the registers held across the sweep loop.
+__device__ __forceinline__ int pinned(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+// word idx of a uniform base, addressed as base + 32-bit byte offset
+__device__ __forceinline__ uint64_t ld_word(const SG_GLOBAL uint64_t *base, int idx) {
+  return *(const SG_GLOBAL uint64_t *)((const SG_GLOBAL char *)base + (uint32_t)pinned(idx * 8));
+}
+__device__ __forceinline__ void st_word(SG_GLOBAL uint64_t *base, int idx, uint64_t v) {
+  *(SG_GLOBAL uint64_t *)((SG_GLOBAL char *)base + (uint32_t)pinned(idx * 8)) = v;
+}
+// device-coherent word access (relaxed agent-scope atomics: sc1, past the
+// per-XCD caches) for the exchange buffer
+__device__ __forceinline__ void st_coherent(uint64_t *base, int idx, uint64_t v) {
+  __hip_atomic_store((uint64_t *)((char *)base + (uint32_t)pinned(idx * 8)), v, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_coherent(const uint64_t *base, int idx) {
+  return __hip_atomic_load((const uint64_t *)((const char *)base + (uint32_t)pinned(idx * 8)),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int L, int P, bool FIELD>
+__global__ __launch_bounds__((SplitGeo<L, P>::NT)) void k_split(const JobPtrs *__restrict__ jt,
+                                                              const TaskDesc *__restrict__ tasks,
+                                                              uint32_t *__restrict__ ticket,
+                                                              int nsweeps, int want_energy) {
+  using G = SplitGeo<L, P>;
+  constexpr int CW = 2 * G::ROWS * G::PL;   // words of one colour's copy
+  static_assert(((G::PU + 1) / 2) * G::NT <= 2 * CW, "fold scratch in the copies");
+  __shared__ uint64_t cb[2 * CW];           // [colour][half][row][column]
+  __shared__ int32_t redi[4 * 64];
+  __shared__ uint32_t s_ticket;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_ticket = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t slot = __builtin_amdgcn_readfirstlane(s_ticket) % gridDim.x;
+  const int ti = (int)(slot / P), w = (int)(slot % P);
+  const TaskDesc tk = ld_uniform(tasks + ti);
+  const JobPtrs jp = ld_uniform(jt + tk.job);
+  const LatDesc d = ld_uniform(jp.descs + tk.lat[0]);
+  SG_GLOBAL uint64_t *__restrict__ g = to_global(jp.spins) + d.spin_off;
+  SG_GLOBAL int32_t *__restrict__ energy = to_global(jp.energy) + (size_t)d.e_idx * 128;
+  const SG_GLOBAL char *__restrict__ Js = (const SG_GLOBAL char *)(to_global(jp.Js) + 2 * d.j_off);
+  uint64_t *xb = jp.xbuf + (size_t)tk.lat[0] * P * G::XW;
+  uint32_t *xf = jp.xflag + (size_t)tk.lat[0] * P;
+  const int wb = (w + P - 1) % P, wa = (w + 1) % P;   // neighbours below, above
+  const int zw = w * G::D;                             // first plane of the lower half
+  // the boundary quad first: threads t < PL own planes 0 (k = 0, boundary)
+  // and 2; the others planes 1 and 3 (k = 1, boundary)
+  const bool upper = __builtin_amdgcn_readfirstlane(tid) >= G::PL;
+
+  // quad k: column col = t mod PL (xh + LH y), plane zl = t / PL + 2 k; site
+  // (hs, c) at z = zw + zl + hs L/2, checkerboard index col + PL z
+  const int col = tid % G::PL;
+  uint64_t sp[2][2][2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int hs = 0; hs < 2; ++hs)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int zl = tid / G::PL + 2 * k;
+        sp[k][hs][c] = ld_word(g, c * G::NH + col + G::PL * (zw + zl + hs * G::LH));
+        cb[c * CW + (hs * G::ROWS + zl + 1) * G::PL + col] = sp[k][hs][c];
+      }
+  // phase 0 reads colour 1: halo planes from the lattice
+  for (int i = tid; i < 4 * G::PL; i += G::NT) {
+    const int hs = i / (2 * G::PL), top = (i / G::PL) & 1, cl = i % G::PL;
+    const int z = (zw + hs * G::LH + (top ? G::D : -1) + L) % L;
+    cb[CW + (hs * G::ROWS + (top ? G::D + 1 : 0)) * G::PL + cl] = ld_word(g, G::NH + cl + G::PL * z);
+  }
+  ThrRec tr = ld_uniform(jp.thr + d.thr_idx);
+  const uint32_t n0 = ld_uniform(jp.js).sweep;
+  if (P > 1 && want_energy && w == 0 && tid < (want_energy > 1 ? 128 : 64))
+    atomicExch((int32_t *)(energy + tid), 0);
+  __syncthreads();
+
+  const int xh = col % G::LH, y = col / G::LH;
+  // the six neighbours of own site (k, hs) at colour c, from the copy of
+  // colour 1 - c: +x, -x, +y, -y, +z, -z (the x neighbour of the same index is
+  // the other colour's register word)
+  auto neighbours = [&](int k, int hs, int c, uint64_t (&nv)[6]) {
+    const int zl = pinned(tid) / G::PL + 2 * k;
+    const int p = (y + zl + zw + hs * G::LH + c) & 1;   // x = 2 xh + p
+    const int row = (1 - c) * CW + (hs * G::ROWS + zl + 1) * G::PL;
+    const int xo = p ? (xh == G::LH - 1 ? -(G::LH - 1) : 1) : (xh == 0 ? G::LH - 1 : -1);
+    const uint64_t xl = cb[row + col + xo], xr = sp[k][hs][1 - c];
+    nv[0] = p ? xl : xr;
+    nv[1] = p ? xr : xl;
+    nv[2] = cb[row + (y == L - 1 ? col - (L - 1) * G::LH : col + G::LH)];
+    nv[3] = cb[row + (y == 0 ? col + (L - 1) * G::LH : col - G::LH)];
+    nv[4] = cb[row + G::PL + col];
+    nv[5] = cb[row - G::PL + col];
+  };
+  auto record = [&](int k, int hs, int c, uint64_t (&jw)[6]) {
+    const int zl = pinned(tid) / G::PL + 2 * k;
+    site_record(Js, c * G::NH + col + G::PL * (zw + zl + hs * G::LH), true, jw);
+  };
+
+  const uint32_t sw = (kTagMC << 28) | (d.rt & 0x0FFFFFFFu);
+  uint32_t r23[2][2];
+  for (int it = 0; it < nsweeps; ++it) {
+    const uint32_t n = n0 + (uint32_t)it;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const uint32_t phase = 2u * n + (uint32_t)c;
+      uint64_t *xme = xb + ((size_t)w * 2 + (phase & 1)) * 4 * G::PL;   // [hs][side][PL]
+      // quad k: update both sites; the new words go to the copy of colour c
+      // (nothing reads it in this phase) and, for the boundary quad, out
+      auto quad = [&](int k, bool boundary) {
+        const int zl = pinned(tid) / G::PL + 2 * k;
+        uint32_t rw[2];
+        if (c == 0) {
+          const uint32_t j = (uint32_t)(col + G::PL * (zw + zl));
+          const u32x4 r = philox_sweep(j, n, d.g, sw, d.key0, d.key1);
+          rw[0] = r.v[0];
+          rw[1] = r.v[1];
+          r23[k][0] = r.v[2];
+          r23[k][1] = r.v[3];
+        } else {
+          rw[0] = r23[k][0];
+          rw[1] = r23[k][1];
+        }
+        uint64_t jc[6], jn[6];
+        record(k, 0, c, jc);
+#pragma unroll
+        for (int hs = 0; hs < 2; ++hs) {
+          if (hs == 0) record(k, 1, c, jn);
+          uint64_t nv[6];
+          neighbours(k, hs, c, nv);
+          update_site<L, FIELD>(sp[k][hs][c], nv, jc, rw[hs], tr);
+#pragma unroll
+          for (int e = 0; e < 6; ++e) jc[e] = jn[e];
+          cb[c * CW + (hs * G::ROWS + zl + 1) * G::PL + col] = sp[k][hs][c];
+          if (P > 1 && boundary)
+            st_coherent(xme, (hs * 2 + (zl == 0 ? 0 : 1)) * G::PL + col, sp[k][hs][c]);
+        }
+      };
+      if (!upper) quad(0, true); else quad(1, true);
+      if (P > 1) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this thread's planes are out
+        __syncthreads();
+        if (tid == 0)
+          __hip_atomic_store(xf + w, phase + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (!upper) quad(1, false); else quad(0, false);
+      if (P > 1) {
+        if (tid == 0) {
+          uint32_t polls = 0;
+          for (;;) {
+            const uint32_t fb = __hip_atomic_load(xf + wb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t fa = __hip_atomic_load(xf + wa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((int32_t)(fb - (phase + 1u)) >= 0 && (int32_t)(fa - (phase + 1u)) >= 0) break;
+            if (++polls > kSplitSpin) {
+              atomicOr(jp.xerr, 1u);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+        }
+        __syncthreads();
+        // halo rows of colour c: below <- the top planes of slab pair wb,
+        // above <- the bottom planes of wa; across the wrap the halves trade
+        // places
+        const uint64_t *xbl = xb + ((size_t)wb * 2 + (phase & 1)) * 4 * G::PL;
+        const uint64_t *xab = xb + ((size_t)wa * 2 + (phase & 1)) * 4 * G::PL;
+        for (int i = tid; i < 4 * G::PL; i += G::NT) {
+          const int hs = i / (2 * G::PL), top = (i / G::PL) & 1, cl = i % G::PL;
+          const uint64_t v =
+              top ? ld_coherent(xab, ((hs ^ (w == P - 1 ? 1 : 0)) * 2 + 0) * G::PL + cl)
+                  : ld_coherent(xbl, ((hs ^ (w == 0 ? 1 : 0)) * 2 + 1) * G::PL + cl);
+          cb[c * CW + (hs * G::ROWS + (top ? G::D + 1 : 0)) * G::PL + cl] = v;
+        }
+      } else {
+        __syncthreads();
+        // one work-group holds the whole lattice: the halo rows of one half
+        // are the other half's end planes
+        for (int i = tid; i < 4 * G::PL; i += G::NT) {
+          const int hs = i / (2 * G::PL), top = (i / G::PL) & 1, cl = i % G::PL;
+          cb[c * CW + (hs * G::ROWS + (top ? G::D + 1 : 0)) * G::PL + cl] =
+              cb[c * CW + ((hs ^ 1) * G::ROWS + (top ? 1 : G::D)) * G::PL + cl];
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // energies of the final states: U over the colour-0 ends (the copy of
+  // colour 1, halos included, is current), K over both colours
+  uint64_t AU[1][G::PU] = {}, AK[1][G::PK] = {};
+  if (want_energy) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int hs = 0; hs < 2; ++hs) {
+        uint64_t jc[6], nv[6];
+        record(k, hs, 0, jc);
+        neighbours(k, hs, 0, nv);
+        uint64_t u0, u1, u2;
+        bond_count(sp[k][hs][0], nv, jc, u0, u1, u2);
+        vc_add3(AU[0], u0, u1, u2);
+        if (want_energy > 1) {
+          vc_add1(AK[0], sp[k][hs][0]);
+          vc_add1(AK[0], sp[k][hs][1]);
+        }
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int hs = 0; hs < 2; ++hs)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int zl = tid / G::PL + 2 * k;
+        st_word(g, c * G::NH + col + G::PL * (zw + zl + hs * G::LH), sp[k][hs][c]);
+      }
+  if (want_energy) {
+    __syncthreads();   // the copies serve as the fold scratch
+    int32_t U[1];
+    wg_count_lanes<G::PU, G::NT, 1>(AU, cb, redi, U);
+    if (tid < 64) {
+      if (P > 1) atomicAdd((int32_t *)(energy + tid), U[0]);
+      else energy[tid] = U[0];
+    }
+    if (want_energy > 1) {
+      __syncthreads();
+      int32_t Kc[1];
+      wg_count_lanes<G::PK, G::NT, 1>(AK, cb, redi, Kc);
+      if (tid < 64) {
+        if (P > 1) atomicAdd((int32_t *)(energy + 64 + tid), Kc[0]);
+        else energy[64 + tid] = Kc[0];
+      }
+    }
+  }
+  // the last work-group out resets the ticket counter for the next launch
+  if (tid == 0 && atomicAdd(ticket + 1, 1u) == gridDim.x - 1) {
+    atomicExch(ticket, 0u);
+    atomicExch(ticket + 1, 0u);
+  }
+}
 
 // ---------------------------------------------------------------- dispatch
 #define SG_LDS_SIZES(X) X(2) X(4) X(6) X(8) X(10) X(12) X(14) X(16)
@@ -2171,7 +2487,7 @@ hipError_t launch_single(int L, bool field, const JobPtrs *jt, const TaskDesc *t
   }
 }
 
-#else  // !SG_RESIDENT_PART
+#elif defined(SG_MAIN_PART)
 
 hipError_t launch_measure(int L, int R, const uint64_t *spins, const int32_t *energy,
                           const MeasDesc *md, int nmd, int32_t *out, int nfield, ObsAcc *acc,
@@ -2308,6 +2624,12 @@ hipError_t launch_build_jsite(int L, uint64_t *Js, const uint64_t *J, uint64_t j
   }
 }
 
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_advance(const JobPtrs *jt, int njobs, int nsweeps, hipStream_t st) {
   if (njobs <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_advance, dim3((njobs + 255) / 256), dim3(256), 0, st, jt, njobs,
@@ -2315,6 +2637,35 @@ hipError_t launch_advance(const JobPtrs *jt, int njobs, int nsweeps, hipStream_t
   return hipGetLastError();
 }
 
-#endif  // SG_RESIDENT_PART
+#endif  // SG_RESIDENT_PART / SG_MAIN_PART
+
+#ifdef SG_SPLIT_PART
+// The split kernels, in a translation unit of their own (sg_kernels_split.hip).
+int split_slabs(int L, int n_lattices) {
+  // L = 32: four slab pairs of 4 + 4 planes per lattice, one work-group per
+  // CU (1024 threads, two quads each, 114 VGPRs), up to 64 lattices
+  return L == 32 && n_lattices * 4 <= 256 ? 4 : 0;
+}
+
+hipError_t launch_split(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int nlat,
+                        int P, uint32_t *ticket, int nsweeps, int want_energy, hipStream_t st) {
+  if (nlat <= 0) return hipSuccess;
+  if (nsweeps < 1) return hipErrorInvalidValue;
+  const dim3 grid(nlat * P);
+#define SG_SPLIT(l, p)                                                                     \
+  if (L == l && P == p) {                                                                 \
+    if (field)                                                                             \
+      hipLaunchKernelGGL((k_split<l, p, true>), grid, dim3((SplitGeo<l, p>::NT)), 0, st, jt,    \
+                         tasks, ticket, nsweeps, want_energy);                             \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_split<l, p, false>), grid, dim3((SplitGeo<l, p>::NT)), 0, st, jt,   \
+                         tasks, ticket, nsweeps, want_energy);                             \
+    return hipGetLastError();                                                              \
+  }
+  SG_SPLIT(32, 4)
+#undef SG_SPLIT
+  return hipErrorInvalidValue;
+}
+#endif  // SG_SPLIT_PART
 
 }  // namespace sg

@@ -19,7 +19,7 @@ using namespace sg::detail;
 namespace {
 struct TuneRes {
   int kernel;
-  double ms[4];   // by SG_KERNEL_* value; 0 where not measured
+  double ms[5];   // by SG_KERNEL_* value; 0 where not measured
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, bool, size_t>, TuneRes> g_tune;
@@ -42,7 +42,8 @@ TuneRes tune_measure(int device, int L, bool field, size_t nlat) {
   std::vector<int> modes = {SG_KERNEL_RESIDENT};
   if (h.can_single) modes.push_back(SG_KERNEL_RESIDENT_SINGLE);
   if (h.can_stream) modes.push_back(SG_KERNEL_STREAMED);
-  TuneRes res{SG_KERNEL_RESIDENT, {0, 0, 0, 0}};
+  if (h.can_split) modes.push_back(SG_KERNEL_SPLIT);
+  TuneRes res{SG_KERNEL_RESIDENT, {0, 0, 0, 0, 0}};
   for (int m : modes) {
     h.set_mode(m);
     h.run(64, nullptr, nullptr);  // warm-up: code, caches, the same captured graphs
@@ -87,14 +88,22 @@ namespace sg {
 namespace detail {
 int tune_kernel_choice(int device, int L, bool field, size_t n_lattices, double *ms) {
   const TuneRes &r = tune_lookup(device, L, field, n_lattices);
-  if (ms) std::copy(r.ms, r.ms + 4, ms);
+  if (ms) std::copy(r.ms, r.ms + 5, ms);
   return r.kernel;
 }
 
 int cached_kernel_choice(int device, int L, bool field, size_t n_lattices) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   auto it = g_tune.find(std::make_tuple(device, L, field, tune_bucket(n_lattices)));
-  return it != g_tune.end() ? it->second.kernel : rule_shape(L, n_lattices);
+  if (it == g_tune.end()) return rule_shape(L, n_lattices);
+  // the engine's classes run concurrently, so they never take the split
+  // shape (its work-groups must run together): the best other shape
+  const TuneRes &r = it->second;
+  if (r.kernel != SG_KERNEL_SPLIT) return r.kernel;
+  int best = SG_KERNEL_RESIDENT;
+  for (int k = 1; k < 4; ++k)
+    if (r.ms[k] > 0 && (r.ms[best] <= 0 || r.ms[k] < r.ms[best])) best = k;
+  return best;
 }
 }  // namespace detail
 }  // namespace sg
@@ -143,10 +152,15 @@ sg_status sg_create(const sg_params *p, sg_handle **out) {
   if (p->pt_every < 0 || p->meas_every < 0) return bad("negative cadence");
   if (!p->betas && !(p->T_min > 0.0 && p->T_max > 0.0)) return bad("need betas or T_min/T_max > 0");
   if (p->form != SG_FORM_PLAIN && p->form != SG_FORM_BATCHED) return bad("unknown form");
-  if (p->kernel < SG_KERNEL_AUTO || p->kernel > SG_KERNEL_RESIDENT_SINGLE)
-    return bad("unknown kernel");
+  if (p->kernel < SG_KERNEL_AUTO || p->kernel > SG_KERNEL_SPLIT) return bad("unknown kernel");
   if (p->kernel == SG_KERNEL_STREAMED && !phase_kernel_supported(p->L))
     return bad("no streamed kernel for this L");
+  if (p->kernel == SG_KERNEL_SPLIT) {
+    const int64_t per = (int64_t)p->n_replicas * p->n_temps;
+    const int64_t n = p->form == SG_FORM_PLAIN ? per : per * ((p->n_samples + 63) / 64);
+    if (n > 4096 || split_slabs(p->L, (int)n) == 0)
+      return bad("no split kernel for this L and size (L = 32, up to 64 lattices per launch)");
+  }
   sg_handle *h = new (std::nothrow) sg_handle();
   if (!h) return bad("host allocation failed");
   try {
@@ -285,7 +299,7 @@ sg_status sg_tune_kernel(int32_t device, int32_t L, int32_t n_lattices, int32_t 
   g_create_error.clear();
   if (!kernel || n_lattices < 1 || L < 2 || L > kMaxL || (L & 1)) return SG_ERR_INVALID;
   try {
-    double ms[4];
+    double ms[5];
     *kernel = tune_kernel_choice(device, L, field != 0, (size_t)n_lattices, ms);
     if (resident_ms) *resident_ms = ms[SG_KERNEL_RESIDENT];
     if (streamed_ms) *streamed_ms = ms[SG_KERNEL_STREAMED];
@@ -297,7 +311,7 @@ sg_status sg_tune_kernel(int32_t device, int32_t L, int32_t n_lattices, int32_t 
 }
 
 sg_status sg_tune_shapes(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
-                         int32_t *kernel, double ms[4]) {
+                         int32_t *kernel, double ms[5]) {
   g_create_error.clear();
   if (!kernel || !ms || n_lattices < 1 || L < 2 || L > kMaxL || (L & 1)) return SG_ERR_INVALID;
   try {

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libsg.so")
 SG_FORM_PLAIN = 0
 SG_FORM_BATCHED = 1
 SG_KERNEL_AUTO, SG_KERNEL_RESIDENT, SG_KERNEL_STREAMED, SG_KERNEL_RESIDENT_SINGLE = 0, 1, 2, 3
+SG_KERNEL_SPLIT = 4
 RES_COLUMNS = ["beta", "e", "m", "q", "q2", "q4", "binder", "chi0", "chi1", "xi_L", "n_meas"]
 NRES = len(RES_COLUMNS)
 
@@ -328,13 +329,14 @@ def tune_kernel(L, n_lattices, field=False, device=0):
 def tune_shapes(L, n_lattices, field=False, device=0):
     """The autotuner's choice and the measured ms of every shape:
     (kernel, {SG_KERNEL_RESIDENT: ms, SG_KERNEL_STREAMED: ms,
-    SG_KERNEL_RESIDENT_SINGLE: ms}), 0 ms where the L has no such shape."""
+    SG_KERNEL_RESIDENT_SINGLE: ms, SG_KERNEL_SPLIT: ms}), 0 ms where the L has
+    no such shape."""
     lib = load_library()
     k = ctypes.c_int32()
-    ms = (ctypes.c_double * 4)()
+    ms = (ctypes.c_double * 5)()
     _check(lib.sg_tune_shapes(device, L, n_lattices, 1 if field else 0, ctypes.byref(k), ms))
     return k.value, {SG_KERNEL_RESIDENT: ms[1], SG_KERNEL_STREAMED: ms[2],
-                     SG_KERNEL_RESIDENT_SINGLE: ms[3]}
+                     SG_KERNEL_RESIDENT_SINGLE: ms[3], SG_KERNEL_SPLIT: ms[4]}
 
 
 STATIC_SCRATCH_BYTES = 32 << 20   # SG_STATIC_SCRATCH_BYTES
