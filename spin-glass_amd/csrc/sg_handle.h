@@ -173,6 +173,13 @@ inline void build_threshold_rec(double b, double H, ThrRec &r) {
   }
   if (H == 0.0 && (r.nfin[0] != 3 || r.nfin[1] != 3))
     throw Fail{SG_ERR_INVALID, "unexpected threshold table at H = 0"};
+  // the sweep kernels count how many thresholds of a spin value the random
+  // number is below, which needs them rising with u (exp(-beta dE) falls
+  // with dE)
+  for (int sig = 0; sig < 2; ++sig)
+    for (uint32_t u = 1; u < r.nfin[sig]; ++u)
+      if (r.t[sig][u - 1] > r.t[sig][u])
+        throw Fail{SG_ERR_INVALID, "thresholds are not monotone in u"};
 }
 
 // Device bytes one engine job needs, block by block as sg_handle::create

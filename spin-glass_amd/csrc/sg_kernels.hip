@@ -33,58 +33,63 @@ constexpr uint32_t kXorOr = (kA ^ (kB | kC)) & 0xFF;                    // a ^ (
 #define SG_BITOP3(a, b, c, imm) __builtin_amdgcn_bitop3_b32((a), (b), (c), (imm))
 
 // One Metropolis update of a 32-lane half word at H = 0: s is the centre
-// half, n[k]/j[k] the six neighbour and coupling halves, R0/R1/R2 all-ones
-// iff the site's random number is below the threshold for u = 0/1/2
-// unsatisfied bonds.  u >= 3 always flips.  18 VALU ops.
+// half, n[k]/j[k] the six neighbour and coupling halves.  A lane flips iff
+// its count u of unsatisfied bonds is at least k, where k = 3 - n and n is the
+// number of thresholds T(u = 2) >= T(u = 1) >= T(u = 0) the site's random
+// number is below (u >= 3 always flips).  u >= k is the carry out of
+// u + (5 + n) >= 8, so the bias 5 + n (bits: P = n even, Q = n in {1, 2},
+// bit 2 = ~R, bit 3 = R = (n == 3), all-ones masks) enters the carry-save
+// tree as extra inputs and only carries are formed: at weight 1,
+// FA(b0..b2) + FA(b3..b5) + P gives carries k1, k2, k3; at weight 2,
+// FA(k1, k2, k3) gives t and m1, and t + Q the carry t & Q; the weight-8
+// carry is maj(m1, t & Q, ~R), and with R it makes m1 | (t & Q) | R.
+// 15 VALU ops.
+constexpr uint32_t kAndOr = ((kA & kB) | kC) & 0xFF;                    // (a & b) | c
 __device__ __forceinline__ uint32_t metropolis32(uint32_t s, const uint32_t (&n)[6],
-                                                 const uint32_t (&j)[6], uint32_t R0,
-                                                 uint32_t R1, uint32_t R2) {
+                                                 const uint32_t (&j)[6], uint32_t P,
+                                                 uint32_t Q, uint32_t R) {
   const uint32_t b0 = SG_BITOP3(s, n[0], j[0], kXor3);
   const uint32_t b1 = SG_BITOP3(s, n[1], j[1], kXor3);
   const uint32_t b2 = SG_BITOP3(s, n[2], j[2], kXor3);
   const uint32_t b3 = SG_BITOP3(s, n[3], j[3], kXor3);
   const uint32_t b4 = SG_BITOP3(s, n[4], j[4], kXor3);
   const uint32_t b5 = SG_BITOP3(s, n[5], j[5], kXor3);
-  const uint32_t s1 = SG_BITOP3(b0, b1, b2, kXor3), c1 = SG_BITOP3(b0, b1, b2, kMaj3);
-  const uint32_t s2 = SG_BITOP3(b3, b4, b5, kXor3), c2 = SG_BITOP3(b3, b4, b5, kMaj3);
-  const uint32_t u0 = s1 ^ s2, c3 = s1 & s2;
-  const uint32_t u1 = SG_BITOP3(c1, c2, c3, kXor3), u2 = SG_BITOP3(c1, c2, c3, kMaj3);
-  const uint32_t lo = SG_BITOP3(u0, R1, R0, kSel);   // u = 1 or 0
-  const uint32_t hi = u0 | R2;                        // u = 3 or 2
-  const uint32_t mid = SG_BITOP3(u1, hi, lo, kSel);
-  return SG_BITOP3(s, u2, mid, kXorOr);               // s ^ (u >= 4 | mid)
+  const uint32_t s1 = SG_BITOP3(b0, b1, b2, kXor3), k1 = SG_BITOP3(b0, b1, b2, kMaj3);
+  const uint32_t s2 = SG_BITOP3(b3, b4, b5, kXor3), k2 = SG_BITOP3(b3, b4, b5, kMaj3);
+  const uint32_t k3 = SG_BITOP3(s1, s2, P, kMaj3);
+  const uint32_t t = SG_BITOP3(k1, k2, k3, kXor3), m1 = SG_BITOP3(k1, k2, k3, kMaj3);
+  const uint32_t g = SG_BITOP3(t, Q, R, kAndOr);
+  return SG_BITOP3(s, m1, g, kXorOr);                 // s ^ (m1 | g)
 }
 
 // The same at H != 0: a lane flips iff rr < T[sigma][u], sigma its spin bit
-// and u its unsatisfied bonds.  R[sigma][u] (u = 0..4) is all-ones iff the
-// site's random number is below T[sigma][u] (or that threshold is 2^32); u =
-// 5, 6 always flip, since dE = 2H sigma' - 8 <= 0 for H <= 4.  25 VALU ops.
-constexpr uint32_t kOr3 = (kA | kB | kC) & 0xFF;                       // a | b | c
-constexpr uint32_t kSelNot = ((kA & ~kB) | (~kA & kC)) & 0xFF;         // a ? ~b : c
+// and u its unsatisfied bonds; u = 5, 6 always flip, since dE = 2H sigma' - 8
+// <= 0 for H <= 4.  T[sigma][u] rises with u, so for each sigma the lane
+// flips iff u >= k_sigma, and u >= k is the carry out of u + (8 - k) >= 8 as
+// at H = 0, with a bias 8 - k in 3..8 whose bits now depend on the lane's
+// spin: C[sigma] = {bit 0, bit 1, bit 2 | bit 3, bit 3} as all-ones masks,
+// each picked by s with one op.  The weight-8 carry is maj(m1, t & bit 1,
+// bit 2), and bit 3 (k = 0: always) forces the flip.  20 VALU ops.
 __device__ __forceinline__ uint32_t metropolis32_field(uint32_t s, const uint32_t (&n)[6],
                                                        const uint32_t (&j)[6],
-                                                       const uint32_t (&R)[2][5]) {
+                                                       const uint32_t (&C)[2][4]) {
   const uint32_t b0 = SG_BITOP3(s, n[0], j[0], kXor3);
   const uint32_t b1 = SG_BITOP3(s, n[1], j[1], kXor3);
   const uint32_t b2 = SG_BITOP3(s, n[2], j[2], kXor3);
   const uint32_t b3 = SG_BITOP3(s, n[3], j[3], kXor3);
   const uint32_t b4 = SG_BITOP3(s, n[4], j[4], kXor3);
   const uint32_t b5 = SG_BITOP3(s, n[5], j[5], kXor3);
-  const uint32_t s1 = SG_BITOP3(b0, b1, b2, kXor3), c1 = SG_BITOP3(b0, b1, b2, kMaj3);
-  const uint32_t s2 = SG_BITOP3(b3, b4, b5, kXor3), c2 = SG_BITOP3(b3, b4, b5, kMaj3);
-  const uint32_t u0 = s1 ^ s2, c3 = s1 & s2;
-  const uint32_t u1 = SG_BITOP3(c1, c2, c3, kXor3), u2 = SG_BITOP3(c1, c2, c3, kMaj3);
-  uint32_t F[2];
-#pragma unroll
-  for (int g = 0; g < 2; ++g) {
-    // u = 0..3 (u2 = 0): R[g][2 u1 + u0]; u = 4..6 (u2 = 1): u = 4 iff u1 = u0 = 0
-    const uint32_t lo = SG_BITOP3(u0, R[g][1], R[g][0], kSel);
-    const uint32_t hi = SG_BITOP3(u0, R[g][3], R[g][2], kSel);
-    const uint32_t m = SG_BITOP3(u1, hi, lo, kSel);
-    const uint32_t t = SG_BITOP3(u1, u0, R[g][4], kOr3);
-    F[g] = SG_BITOP3(u2, t, m, kSel);
-  }
-  return SG_BITOP3(s, F[1], F[0], kSelNot);   // s = 1: s ^ F1 = ~F1; s = 0: F0
+  const uint32_t s1 = SG_BITOP3(b0, b1, b2, kXor3), k1 = SG_BITOP3(b0, b1, b2, kMaj3);
+  const uint32_t s2 = SG_BITOP3(b3, b4, b5, kXor3), k2 = SG_BITOP3(b3, b4, b5, kMaj3);
+  const uint32_t c0 = SG_BITOP3(s, C[1][0], C[0][0], kSel);
+  const uint32_t k3 = SG_BITOP3(s1, s2, c0, kMaj3);
+  const uint32_t t = SG_BITOP3(k1, k2, k3, kXor3), m1 = SG_BITOP3(k1, k2, k3, kMaj3);
+  const uint32_t c1 = SG_BITOP3(s, C[1][1], C[0][1], kSel);
+  const uint32_t m2 = t & c1;
+  const uint32_t d2 = SG_BITOP3(s, C[1][2], C[0][2], kSel);
+  const uint32_t w8 = SG_BITOP3(m1, m2, d2, kMaj3);
+  const uint32_t c3 = SG_BITOP3(s, C[1][3], C[0][3], kSel);
+  return SG_BITOP3(s, w8, c3, kXorOr);                 // s ^ (w8 | c3)
 }
 
 // Philox4x32-10 with the key XORs as single 3-input ops (same numbers as
@@ -505,10 +510,12 @@ __device__ __forceinline__ void update_site(uint64_t &s, const uint64_t (&nv)[6]
                                             const uint64_t (&jw)[6], uint32_t rr,
                                             const ThrRec &tr) {
   if (!FIELD) {
-    // u >= 3 always flips; u = 2, 1, 0 flip iff rr < T(u)
-    const uint32_t R0 = rr < tr.t[0][0] ? ~0u : 0u;
-    const uint32_t R1 = rr < tr.t[0][1] ? ~0u : 0u;
-    const uint32_t R2 = rr < tr.t[0][2] ? ~0u : 0u;
+    // u >= 3 always flips; u = 2, 1, 0 flip iff rr < T(u).  The host keeps
+    // T(0) <= T(1) <= T(2), so below T(0) implies below T(1) and T(2).
+    const bool a0 = rr < tr.t[0][0], a1 = rr < tr.t[0][1], a2 = rr < tr.t[0][2];
+    const uint32_t P = (a0 ^ a1 ^ a2) ? 0u : ~0u;   // the count of a's is even
+    const uint32_t Q = (a2 != a0) ? ~0u : 0u;        // ... is 1 or 2 (a0 implies a2)
+    const uint32_t R = a0 ? ~0u : 0u;                // ... is 3
     uint32_t nlo[6], nhi[6], jlo[6], jhi[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -517,17 +524,26 @@ __device__ __forceinline__ void update_site(uint64_t &s, const uint64_t (&nv)[6]
       jlo[k] = (uint32_t)jw[k];
       jhi[k] = (uint32_t)(jw[k] >> 32);
     }
-    const uint32_t lo = metropolis32((uint32_t)s, nlo, jlo, R0, R1, R2);
-    const uint32_t hi = metropolis32((uint32_t)(s >> 32), nhi, jhi, R0, R1, R2);
+    const uint32_t lo = metropolis32((uint32_t)s, nlo, jlo, P, Q, R);
+    const uint32_t hi = metropolis32((uint32_t)(s >> 32), nhi, jhi, P, Q, R);
     s = (uint64_t)lo | ((uint64_t)hi << 32);
   } else {
-    // thresholds past nfin are 2^32: always below (the host keeps nfin <= 5)
-    uint32_t R[2][5];
+    // a[u] = rr is below T[sigma][u] (thresholds past nfin are 2^32: always
+    // below; the host keeps nfin <= 5 and T rising in u): a thermometer code,
+    // a[u] implies a[u + 1], of m = a[0] + ... + a[4] = 5 - k.  The bias
+    // 8 - k = 3 + m has bit 0 = m even, bit 1 = m in {0, 3, 4}, bit 2 | bit 3
+    // = m >= 1 and bit 3 = m == 5.
+    uint32_t C[2][4];
 #pragma unroll
-    for (int g = 0; g < 2; ++g)
+    for (int g = 0; g < 2; ++g) {
+      bool a[5];
 #pragma unroll
-      for (int u = 0; u < 5; ++u)
-        R[g][u] = ((uint32_t)u >= tr.nfin[g] || rr < tr.t[g][u]) ? ~0u : 0u;
+      for (int u = 0; u < 5; ++u) a[u] = (uint32_t)u >= tr.nfin[g] || rr < tr.t[g][u];
+      C[g][0] = (a[0] ^ a[1] ^ a[2] ^ a[3] ^ a[4]) ? 0u : ~0u;
+      C[g][1] = (a[4] == a[2] && !a[0]) ? ~0u : 0u;
+      C[g][2] = a[4] ? ~0u : 0u;
+      C[g][3] = a[0] ? ~0u : 0u;
+    }
     uint32_t nlo[6], nhi[6], jlo[6], jhi[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -536,8 +552,8 @@ __device__ __forceinline__ void update_site(uint64_t &s, const uint64_t (&nv)[6]
       jlo[k] = (uint32_t)jw[k];
       jhi[k] = (uint32_t)(jw[k] >> 32);
     }
-    const uint32_t lo = metropolis32_field((uint32_t)s, nlo, jlo, R);
-    const uint32_t hi = metropolis32_field((uint32_t)(s >> 32), nhi, jhi, R);
+    const uint32_t lo = metropolis32_field((uint32_t)s, nlo, jlo, C);
+    const uint32_t hi = metropolis32_field((uint32_t)(s >> 32), nhi, jhi, C);
     s = (uint64_t)lo | ((uint64_t)hi << 32);
   }
 }

@@ -316,8 +316,8 @@ class Sim:
 
 def tune_kernel(L, n_lattices, field=False, device=0):
     """The autotuner's choice for n_lattices lattices of size L:
-    (SG_KERNEL_RESIDENT, _STREAMED or _RESIDENT_SINGLE, lattice-pair ms,
-    streamed ms)."""
+    (SG_KERNEL_RESIDENT, _STREAMED, _RESIDENT_SINGLE or _SPLIT, lattice-pair
+    ms, streamed ms)."""
     lib = load_library()
     k = ctypes.c_int32()
     r, st = ctypes.c_double(), ctypes.c_double()

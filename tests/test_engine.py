@@ -213,14 +213,15 @@ def test_job_footprint_counts_every_block():
 @pytest.mark.gpu
 def test_autotuner_measures_every_shape_and_caches():
     P, S, O = sg.SG_KERNEL_RESIDENT, sg.SG_KERNEL_STREAMED, sg.SG_KERNEL_RESIDENT_SINGLE
+    X = sg.SG_KERNEL_SPLIT
     k, ms = sg.tune_shapes(8, 64)
-    assert ms[P] > 0 and ms[O] > 0 and ms[S] == 0                # no streamed kernel below L=12
+    assert ms[P] > 0 and ms[O] > 0 and ms[S] == 0 and ms[X] == 0  # no streamed kernel below L=12
     assert k == min((P, O), key=lambda m: ms[m])
     k, ms = sg.tune_shapes(24, 64)                               # k_one: one shape resident
-    assert ms[P] > 0 and ms[O] == 0 and ms[S] > 0 and k in (P, S)
-    k, ms = sg.tune_shapes(32, 64)
-    assert ms[P] > 0 and ms[O] > 0 and ms[S] > 0
-    assert k == min((P, O, S), key=lambda m: ms[m])
+    assert ms[P] > 0 and ms[O] == 0 and ms[S] > 0 and ms[X] == 0 and k in (P, S)
+    k, ms = sg.tune_shapes(32, 64)                               # split: 4 x 64 work-groups fit
+    assert ms[P] > 0 and ms[O] > 0 and ms[S] > 0 and ms[X] > 0
+    assert k == min((P, O, S, X), key=lambda m: ms[m])
     assert sg.tune_shapes(32, 60) == (k, ms)                     # same power-of-two bucket
     assert sg.tune_kernel(32, 64) == (k, ms[P], ms[S])
 
