@@ -297,6 +297,12 @@ sg_status sg_batch_stats(sg_batch *b, double *flips, int64_t *steps, int64_t *ar
  * graphs of a class are dropped when a job joins or leaves it. */
 sg_status sg_batch_launch_stats(sg_batch *b, int64_t *intervals_direct,
                                 int64_t *intervals_graphed, int64_t *graphs_built);
+/* Host-side allocations of the engine beyond its arena: growths of the
+ * classes' task tables (each a hipMalloc of twice the size; outgrown tables
+ * are freed with the batch, never while it runs) and pinned blocks allocated
+ * for the jobs' final copies (retired jobs give theirs back for reuse).  No
+ * counterpart in the reference. */
+sg_status sg_batch_alloc_stats(sg_batch *b, int64_t *table_growths, int64_t *pinned_blocks);
 
 /* Serving loop: jobs[i] arrives arrival_s[i] seconds after the call starts
  * (non-decreasing); each is submitted when the host clock reaches its

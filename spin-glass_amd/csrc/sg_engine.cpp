@@ -23,6 +23,46 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Pinned host blocks for the jobs' final copies, kept for reuse: a retired
+// job gives its blocks back here rather than to hipHostFree (which
+// synchronises the device); sizes are powers of two from 4 KiB.  Freed with
+// the batch.
+class HostPool {
+ public:
+  HostPool() = default;
+  HostPool(const HostPool &) = delete;
+  HostPool &operator=(const HostPool &) = delete;
+  ~HostPool() {
+    for (auto &b : free_) hipHostFree(b.second);
+  }
+  static size_t size_class(size_t n) {
+    size_t c = 4096;
+    while (c < n) c <<= 1;
+    return c;
+  }
+  void *take(size_t n) {
+    const size_t c = size_class(n);
+    auto it = free_.find(c);
+    if (it != free_.end()) {
+      void *p = it->second;
+      free_.erase(it);
+      return p;
+    }
+    void *p = nullptr;
+    HIPCHK(hipHostMalloc(&p, c));
+    ++fresh_;
+    return p;
+  }
+  void give(void *p, size_t n) {
+    if (p) free_.emplace(size_class(n), p);
+  }
+  int64_t fresh() const { return fresh_; }   // blocks allocated so far
+
+ private:
+  std::multimap<size_t, void *> free_;
+  int64_t fresh_ = 0;
+};
+
 struct Job {
   int64_t id = -1;
   sg_params p{};
@@ -35,14 +75,18 @@ struct Job {
   std::unique_ptr<sg_handle> h;
   ObsAcc *h_acc = nullptr;  // pinned copies of the final sums and counters
   uint64_t *h_cnt = nullptr;
+  size_t acc_bytes = 0, cnt_bytes = 0;
+  HostPool *pool = nullptr;  // where h_acc and h_cnt came from
   hipEvent_t done_ev = nullptr;
   std::vector<double> res;
   std::vector<uint64_t> swaps;
   double t_submit = 0, t_admit = 0, t_done = 0;
 
   void free_host() {
-    if (h_acc) hipHostFree(h_acc);
-    if (h_cnt) hipHostFree(h_cnt);
+    if (pool) {
+      pool->give(h_acc, acc_bytes);
+      pool->give(h_cnt, cnt_bytes);
+    }
     if (done_ev) hipEventDestroy(done_ev);
     h_acc = nullptr;
     h_cnt = nullptr;
@@ -51,7 +95,24 @@ struct Job {
   ~Job() { free_host(); }
 };
 
-// A device array with pinned staging, grown on demand.
+// A device array with pinned staging, grown on demand by doubling.  The
+// arrays it outgrows go to a graveyard (launches still queued may read them)
+// and are freed with their class, so growing never waits for a stream and
+// never calls hipFree (which synchronises the device) while jobs run.
+struct Grave {
+  std::vector<std::pair<void *, void *>> arrays;  // (device, pinned host)
+  void bury(void *d, void *h) {
+    if (d || h) arrays.emplace_back(d, h);
+  }
+  void free_all() {
+    for (auto &a : arrays) {
+      if (a.first) hipFree(a.first);
+      if (a.second) hipHostFree(a.second);
+    }
+    arrays.clear();
+  }
+};
+
 template <class T>
 struct DevList {
   T *d = nullptr, *h = nullptr;
@@ -90,6 +151,8 @@ struct Class {
   DevList<TaskDesc> ptc[2];     // streamed: (pair, chunk) tasks of the PT launch
   hipEvent_t up_ev = nullptr;   // the last upload from the staging
   bool uploaded = false;
+  Grave grave;                  // outgrown table arrays
+  int64_t grown = 0;            // table growths so far
   // Batched intervals captured as hipGraphs: one per (first parity list,
   // intervals, energy mode of the last interval), valid while the tables
   // above are.  Graphs of earlier tables wait in `old` until the event
@@ -124,6 +187,7 @@ struct Class {
     if (st) hipStreamSynchronize(st);
     retire_graphs();
     reap_graphs(true);
+    grave.free_all();
     jt.free_all();
     chunks.free_all();
     slabs.free_all();
@@ -142,6 +206,7 @@ struct sg_batch {
   std::string err;
   sg_batch_params bp{};
   DevArena arena;
+  HostPool host_pool;   // declared before jobs: they give their blocks back to it
   Clock::time_point t0;
   std::map<int64_t, std::unique_ptr<Job>> jobs;
   std::deque<Job *> queue;            // submitted, not admitted
@@ -256,8 +321,11 @@ struct sg_batch {
     h->init_lattice();
     h->launch_energy_all(h->energy_mode());
     const size_t nc = (size_t)std::max(j.p.n_temps - 1, 1);
-    HIPCHK(hipHostMalloc((void **)&j.h_acc, sizeof(ObsAcc) * (size_t)h->G * h->T * 64));
-    HIPCHK(hipHostMalloc((void **)&j.h_cnt, sizeof(uint64_t) * nc));
+    j.acc_bytes = sizeof(ObsAcc) * (size_t)h->G * h->T * 64;
+    j.cnt_bytes = sizeof(uint64_t) * nc;
+    j.pool = &host_pool;
+    j.h_acc = (ObsAcc *)host_pool.take(j.acc_bytes);
+    j.h_cnt = (uint64_t *)host_pool.take(j.cnt_bytes);
     HIPCHK(hipEventCreateWithFlags(&j.done_ev, hipEventDisableTiming));
     j.h = std::move(h);
     j.cls = class_index(c);
@@ -312,16 +380,18 @@ struct sg_batch {
   }
 
   // ------------------------------------------------------------ a class's tables
-  // Grows a list so it holds n entries; queued launches may still read the
-  // old device array, so growth waits for the class's stream.
+  // Grows a list so it holds n entries.  Queued launches may still read the
+  // old device array and a queued upload the old staging, so both go to the
+  // class's graveyard rather than being freed.
   template <class T>
   void reserve(Class &c, DevList<T> &l, size_t n) {
     if (n <= l.cap && l.d) return;
-    HIPCHK(hipStreamSynchronize(c.st));
-    l.free_all();
+    c.grave.bury(l.d, l.h);
+    l.d = l.h = nullptr;
     l.cap = std::max<size_t>(2 * n, 64);
     HIPCHK(hipMalloc((void **)&l.d, sizeof(T) * l.cap));
     HIPCHK(hipHostMalloc((void **)&l.h, sizeof(T) * l.cap));
+    ++c.grown;
   }
 
   template <class T>
@@ -747,6 +817,15 @@ sg_status sg_batch_launch_stats(sg_batch *b, int64_t *intervals_direct,
     if (intervals_direct) *intervals_direct = b->iv_direct;
     if (intervals_graphed) *intervals_graphed = b->iv_graphed;
     if (graphs_built) *graphs_built = b->graphs_built;
+  });
+}
+
+sg_status sg_batch_alloc_stats(sg_batch *b, int64_t *table_growths, int64_t *pinned_blocks) {
+  return bguarded(b, [&] {
+    int64_t g = 0;
+    for (auto &c : b->classes) g += c.grown;
+    if (table_growths) *table_growths = g;
+    if (pinned_blocks) *pinned_blocks = b->host_pool.fresh();
   });
 }
 

@@ -59,7 +59,7 @@ EXPORTS = [
     "sg_get_timing", "sg_set_profiling", "sg_get_kernel_stats", "sg_tune_kernel", "sg_tune_shapes",
     "sg_batch_params_default", "sg_batch_create", "sg_batch_destroy", "sg_batch_last_error",
     "sg_job_footprint", "sg_submit", "sg_step", "sg_drain", "sg_poll", "sg_result",
-    "sg_job_times", "sg_batch_stats", "sg_batch_launch_stats", "sg_serve",
+    "sg_job_times", "sg_batch_stats", "sg_batch_launch_stats", "sg_batch_alloc_stats", "sg_serve",
     "sg_probe_valu", "sg_probe_last_error", "sg_prepare", "sg_graph_count",
     "sg_tune_count", "sg_run_static",
 ]
@@ -132,6 +132,7 @@ def load_library(path: str = LIB_PATH):
     L.sg_job_times.argtypes = [_vp, ctypes.c_int64, _dp, _dp, _dp]
     L.sg_batch_stats.argtypes = [_vp, _dp, _i64p, _i64p, _i64p]
     L.sg_batch_launch_stats.argtypes = [_vp, _i64p, _i64p, _i64p]
+    L.sg_batch_alloc_stats.argtypes = [_vp, _i64p, _i64p]
     L.sg_serve.argtypes = [_vp, ctypes.POINTER(_Params), _i64p, _dp, ctypes.c_int32, _i64p, _dp]
     L.sg_run_static.argtypes = [ctypes.POINTER(_Params), _vp, _vp, ctypes.c_int64, ctypes.c_int64,
                                 _vp, _vp, _vp]
@@ -505,6 +506,13 @@ class Batch:
                                                      ctypes.byref(n)))
         return {"intervals_direct": d.value, "intervals_graphed": g.value,
                 "graphs_built": n.value}
+
+    def alloc_stats(self):
+        """Allocations beyond the arena: task-table growths and pinned
+        blocks allocated for final copies (reused after retirement)."""
+        g, b = ctypes.c_int64(), ctypes.c_int64()
+        self._c(load_library().sg_batch_alloc_stats(self._b, ctypes.byref(g), ctypes.byref(b)))
+        return {"table_growths": g.value, "pinned_blocks": b.value}
 
     def serve(self, jobs, arrivals):
         """Runs the native serving loop over jobs arriving at `arrivals`

@@ -282,6 +282,27 @@ def _serving_stream(n, rate, sweeps_lo, sweeps_hi, seed):
 
 
 @pytest.mark.gpu
+def test_second_wave_of_jobs_allocates_nothing():
+    """Twelve serving-shaped jobs submitted at once, served to completion, then
+    the same twelve again on the same engine: the second wave grows no task
+    table and allocates no pinned block (retired jobs gave theirs back), and
+    its results and swap counts equal the first wave's bit for bit."""
+    jobs, _ = _serving_stream(12, 200.0, 64, 256, 0x5EED0006)
+    b = sg.Batch(pt_every=PT, slice_pt_steps=2, arena_bytes=4 << 30)
+    ids, _ = b.serve(jobs, [0.0] * len(jobs))
+    first = [b.result(i) for i in ids]
+    a1 = b.alloc_stats()
+    assert a1["pinned_blocks"] > 0 and a1["table_growths"] > 0
+    ids, _ = b.serve(jobs, [0.0] * len(jobs))
+    second = [b.result(i) for i in ids]
+    assert b.alloc_stats() == a1
+    b.close()
+    for j, (r1, c1), (r2, c2) in zip(jobs, first, second):
+        np.testing.assert_array_equal(r1, r2, err_msg=f"L={j.L}")
+        np.testing.assert_array_equal(c1, c2, err_msg=f"L={j.L}")
+
+
+@pytest.mark.gpu
 def test_serving_workload_jobs_equal_their_alone_runs():
     """The serving configuration, scaled to test time: 24 jobs arriving as a
     Poisson stream at 200/s (seed 0x5EED0005), L from {8..24}, 16-32
