@@ -37,17 +37,17 @@ constexpr uint32_t kXorOr = (kA ^ (kB | kC)) & 0xFF;                    // a ^ (
 // its count u of unsatisfied bonds is at least k, where k = 3 - n and n is the
 // number of thresholds T(u = 2) >= T(u = 1) >= T(u = 0) the site's random
 // number is below (u >= 3 always flips).  u >= k is the carry out of
-// u + (5 + n) >= 8, so the bias 5 + n (bits: P = n even, Q = n in {1, 2},
-// bit 2 = ~R, bit 3 = R = (n == 3), all-ones masks) enters the carry-save
-// tree as extra inputs and only carries are formed: at weight 1,
-// FA(b0..b2) + FA(b3..b5) + P gives carries k1, k2, k3; at weight 2,
-// FA(k1, k2, k3) gives t and m1, and t + Q the carry t & Q; the weight-8
-// carry is maj(m1, t & Q, ~R), and with R it makes m1 | (t & Q) | R.
-// 15 VALU ops.
-constexpr uint32_t kAndOr = ((kA & kB) | kC) & 0xFF;                    // (a & b) | c
+// u + (5 + n) >= 8, so the bias 5 + n enters the carry-save tree as extra
+// inputs and only carries are formed.  With B0 = n odd and B1 = n >= 2
+// (all-ones masks) the bias bits are ~B0 (weight 1), B0 ^ B1 (weight 2),
+// ~(B0 & B1) (weight 4) and B0 & B1 (weight 8): at weight 1, FA(b0..b2) +
+// FA(b3..b5) + ~B0 gives carries k1, k2, k3; at weight 2, FA(k1, k2, k3)
+// gives t and m1; the weight-8 carry with the weight-8 bias bit is then
+// m1 | (t & (B0 ^ B1)) | (B0 & B1) = m1 | maj(t, B0, B1).  15 VALU ops.
+constexpr uint32_t kMaj3NotC = ((kA & kB) | (kA & ~kC) | (kB & ~kC)) & 0xFF;  // maj(a, b, ~c)
 __device__ __forceinline__ uint32_t metropolis32(uint32_t s, const uint32_t (&n)[6],
-                                                 const uint32_t (&j)[6], uint32_t P,
-                                                 uint32_t Q, uint32_t R) {
+                                                 const uint32_t (&j)[6], uint32_t B0,
+                                                 uint32_t B1) {
   const uint32_t b0 = SG_BITOP3(s, n[0], j[0], kXor3);
   const uint32_t b1 = SG_BITOP3(s, n[1], j[1], kXor3);
   const uint32_t b2 = SG_BITOP3(s, n[2], j[2], kXor3);
@@ -56,9 +56,9 @@ __device__ __forceinline__ uint32_t metropolis32(uint32_t s, const uint32_t (&n)
   const uint32_t b5 = SG_BITOP3(s, n[5], j[5], kXor3);
   const uint32_t s1 = SG_BITOP3(b0, b1, b2, kXor3), k1 = SG_BITOP3(b0, b1, b2, kMaj3);
   const uint32_t s2 = SG_BITOP3(b3, b4, b5, kXor3), k2 = SG_BITOP3(b3, b4, b5, kMaj3);
-  const uint32_t k3 = SG_BITOP3(s1, s2, P, kMaj3);
+  const uint32_t k3 = SG_BITOP3(s1, s2, B0, kMaj3NotC);
   const uint32_t t = SG_BITOP3(k1, k2, k3, kXor3), m1 = SG_BITOP3(k1, k2, k3, kMaj3);
-  const uint32_t g = SG_BITOP3(t, Q, R, kAndOr);
+  const uint32_t g = SG_BITOP3(t, B0, B1, kMaj3);
   return SG_BITOP3(s, m1, g, kXorOr);                 // s ^ (m1 | g)
 }
 
@@ -513,9 +513,8 @@ __device__ __forceinline__ void update_site(uint64_t &s, const uint64_t (&nv)[6]
     // u >= 3 always flips; u = 2, 1, 0 flip iff rr < T(u).  The host keeps
     // T(0) <= T(1) <= T(2), so below T(0) implies below T(1) and T(2).
     const bool a0 = rr < tr.t[0][0], a1 = rr < tr.t[0][1], a2 = rr < tr.t[0][2];
-    const uint32_t P = (a0 ^ a1 ^ a2) ? 0u : ~0u;   // the count of a's is even
-    const uint32_t Q = (a2 != a0) ? ~0u : 0u;        // ... is 1 or 2 (a0 implies a2)
-    const uint32_t R = a0 ? ~0u : 0u;                // ... is 3
+    const uint32_t B0 = (a0 ^ a1 ^ a2) ? ~0u : 0u;   // the count of a's is odd
+    const uint32_t B1 = a1 ? ~0u : 0u;               // ... is 2 or 3
     uint32_t nlo[6], nhi[6], jlo[6], jhi[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -524,8 +523,8 @@ __device__ __forceinline__ void update_site(uint64_t &s, const uint64_t (&nv)[6]
       jlo[k] = (uint32_t)jw[k];
       jhi[k] = (uint32_t)(jw[k] >> 32);
     }
-    const uint32_t lo = metropolis32((uint32_t)s, nlo, jlo, P, Q, R);
-    const uint32_t hi = metropolis32((uint32_t)(s >> 32), nhi, jhi, P, Q, R);
+    const uint32_t lo = metropolis32((uint32_t)s, nlo, jlo, B0, B1);
+    const uint32_t hi = metropolis32((uint32_t)(s >> 32), nhi, jhi, B0, B1);
     s = (uint64_t)lo | ((uint64_t)hi << 32);
   } else {
     // a[u] = rr is below T[sigma][u] (thresholds past nfin are 2^32: always

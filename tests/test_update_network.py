@@ -30,13 +30,14 @@ def rule(u, s, R):
     return 1 if u >= 5 else R[s][u]
 
 
-def carry_network_h0(e, P, Q, R):
-    """metropolis32: 15 ops per half word (the 6 bond XORs give e)."""
+def carry_network_h0(e, B0, B1):
+    """metropolis32: 15 ops per half word (the 6 bond XORs give e); B0 = the
+    threshold count n is odd, B1 = n >= 2."""
     s1, k1 = e[0] ^ e[1] ^ e[2], maj(e[0], e[1], e[2])
     s2, k2 = e[3] ^ e[4] ^ e[5], maj(e[3], e[4], e[5])
-    k3 = maj(s1, s2, P)
+    k3 = maj(s1, s2, 1 - B0)
     t, m1 = k1 ^ k2 ^ k3, maj(k1, k2, k3)
-    g = (t & Q) | R
+    g = maj(t, B0, B1)
     return m1 | g
 
 
@@ -62,11 +63,10 @@ def test_h0_network_equals_the_threshold_rule():
         for n in range(4):
             a0, a1, a2 = int(n >= 3), int(n >= 2), int(n >= 1)
             R_rule = [[a0, a1, a2, 1, 1]] * 2        # u = 3, 4 always flip
-            P = 1 - (a0 ^ a1 ^ a2)                   # the kernel's masks
-            Q = a2 ^ a0
-            R = a0
+            B0 = a0 ^ a1 ^ a2                        # the kernel's masks
+            B1 = a1
             for s in (0, 1):
-                assert carry_network_h0(e, P, Q, R) == rule(u, s, R_rule), (e, n)
+                assert carry_network_h0(e, B0, B1) == rule(u, s, R_rule), (e, n)
                 checked += 1
     assert checked == 64 * 4 * 2
 
@@ -93,12 +93,12 @@ def test_field_network_equals_the_threshold_rule():
 
 
 def test_a_dropped_bias_bit_is_caught():
-    """The enumeration is sharp: the H = 0 network without its weight-1
-    bias (P = 0) disagrees with the rule somewhere."""
+    """The enumeration is sharp: the H = 0 network with its weight-1 bias
+    bit inverted (B0 taken for n even) disagrees with the rule somewhere."""
     bad = 0
     for e in itertools.product((0, 1), repeat=6):
         for n in range(4):
             a0, a1, a2 = int(n >= 3), int(n >= 2), int(n >= 1)
             R_rule = [[a0, a1, a2, 1, 1]] * 2
-            bad += carry_network_h0(e, 0, a2 ^ a0, a0) != rule(sum(e), 0, R_rule)
+            bad += carry_network_h0(e, 1 - (a0 ^ a1 ^ a2), a1) != rule(sum(e), 0, R_rule)
     assert bad > 0
