@@ -1767,7 +1767,7 @@ __global__ __launch_bounds__(256) void k_build_jsite(uint64_t *__restrict__ Js,
 // U (and K) of the states after this launch (colour 1 only: every bond has
 // one white end, every site index h one white and one black word) into the
 // slots with per-lane atomics.
-template <int L, bool FIELD>
+template <int L, bool FIELD, bool ENG>
 __global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ jt,
                                                     const TaskDesc *__restrict__ tasks, int c,
                                                     int it, int mode) {
@@ -1788,7 +1788,7 @@ __global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ 
   const SG_GLOBAL uint64_t *__restrict__ Js =
       to_global(jp.Js) + 2 * d.j_off + (uint64_t)c * G::NH * 6;
   const bool upd = (mode & kPhUpdate) != 0;
-  const bool eng = (mode & (kPhEnergy | kPhEnergyK)) != 0;
+  const bool eng = ENG && (mode & (kPhEnergy | kPhEnergyK)) != 0;
   // quad j = xh + (L/2)(y + L z) with z < L/2; its sites are (xh, y, z) and
   // (xh, y, z + L/2): one column, one set of divisions.  Every load is
   // issued before the Philox rounds, which then hide its latency.
@@ -1819,7 +1819,7 @@ __global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ 
       nv[q][4] = opp[xh + ry + pzp];
       nv[q][5] = opp[xh + ry + pzm];
       sq[q] = mine[h];
-      if (mode & kPhEnergyK) bq[q] = opp[h];
+      if (ENG && (mode & kPhEnergyK)) bq[q] = opp[h];
     }
   }
   // the quad's Philox block, drawn by both colour launches of the sweep:
@@ -1850,7 +1850,7 @@ __global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ 
         uint64_t u0, u1, u2;
         bond_count(s, nv[q], jw[q], u0, u1, u2);
         vc_add3(AU[0], u0, u1, u2);
-        if (mode & kPhEnergyK) {
+        if (ENG && (mode & kPhEnergyK)) {
           vc_add1(AK[0], s);
           vc_add1(AK[0], bq[q]);
         }
@@ -1861,7 +1861,7 @@ __global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ 
     int32_t U[1];
     wg_count_lanes<4, kPhaseNT, 1>(AU, fold, redi, U);
     if (tid < 64) atomicAdd((int32_t *)(energy + tid), U[0]);
-    if (mode & kPhEnergyK) {
+    if (ENG && (mode & kPhEnergyK)) {
       __syncthreads();
       int32_t K[1];
       wg_count_lanes<3, kPhaseNT, 1>(AK, fold, redi, K);
@@ -2564,15 +2564,26 @@ bool phase_kernel_supported(int L) {
 hipError_t launch_phase(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
                         int c, int it, int mode, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
+  // launches that accumulate energies take the instance with the counting
+  // code; the update-only instance holds fewer registers
+  const bool eng = (mode & (kPhEnergy | kPhEnergyK)) != 0;
   switch (L) {
 #define X(l)                                                                               \
   case l:                                                                                  \
-    if (field)                                                                             \
-      hipLaunchKernelGGL((k_phase<l, true>), dim3(ntasks), dim3(kPhaseNT), 0, st, jt, tasks, \
-                         c, it, mode);                                                     \
-    else                                                                                   \
-      hipLaunchKernelGGL((k_phase<l, false>), dim3(ntasks), dim3(kPhaseNT), 0, st, jt,      \
-                         tasks, c, it, mode);                                              \
+    if (eng) {                                                                             \
+      if (field)                                                                           \
+        hipLaunchKernelGGL((k_phase<l, true, true>), dim3(ntasks), dim3(kPhaseNT), 0, st,   \
+                           jt, tasks, c, it, mode);                                        \
+      else                                                                                 \
+        hipLaunchKernelGGL((k_phase<l, false, true>), dim3(ntasks), dim3(kPhaseNT), 0, st,  \
+                           jt, tasks, c, it, mode);                                        \
+    } else if (field) {                                                                    \
+      hipLaunchKernelGGL((k_phase<l, true, false>), dim3(ntasks), dim3(kPhaseNT), 0, st,    \
+                         jt, tasks, c, it, mode);                                          \
+    } else {                                                                               \
+      hipLaunchKernelGGL((k_phase<l, false, false>), dim3(ntasks), dim3(kPhaseNT), 0, st,   \
+                         jt, tasks, c, it, mode);                                          \
+    }                                                                                      \
     return hipGetLastError();
     SG_PHASE_SIZES(X)
 #undef X
