@@ -63,7 +63,7 @@ TRAFFIC_L16 = 109.4e6
 # VALU wave-instructions per k_block<16,false> launch of the l16 workload
 # (SQ_INSTS_VALU, profiles/k_block_L16_batched_2048_counters.txt): x 64 lanes
 # over the launch duration gives the issued lane-ops/s.
-VALU_WAVE_INSTS_L16 = 79.05e6
+VALU_WAVE_INSTS_L16 = 77.47e6
 
 
 def make(samples, form):
