@@ -108,11 +108,13 @@ def test_split_kernel_matches_oracle(H, form):
 
 @pytest.mark.parametrize("kernel", [sg.SG_KERNEL_RESIDENT, sg.SG_KERNEL_STREAMED])
 @pytest.mark.parametrize("H", [0.0, 0.5])
-@pytest.mark.parametrize("L", [30, 36, 46])
+@pytest.mark.parametrize("L", [30, 34, 36, 38, 42, 44, 46])
 def test_other_large_sizes_match_oracle(L, H, kernel):
     """Sizes the benchmarks do not use: k_big (resident, in L2) and k_phase
-    (streamed colour launches) at L = 30 and 46 (L = 2 mod 4: a colour row
-    holds an odd number of words) and 36, at H = 0 and H != 0."""
+    (streamed colour launches) at L = 30, 34, 38, 42 and 46 (L = 2 mod 4: a
+    colour row holds an odd number of words) and 36, 44, at H = 0 and
+    H != 0.  With L = 28, 32, 40, 48 elsewhere, every even L above 26 that
+    sg_create accepts is covered."""
     betas = geometric_betas(0.3, 2.0, 3)
     sim, orc = _pair(L, 64, 2, betas, H=H, pt_every=2, meas_every=3, kernel=kernel)
     sim.run(4)
