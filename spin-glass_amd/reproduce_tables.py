@@ -64,34 +64,10 @@ def run(L, H, samples, sweeps, out_dir, seed0=0x5EED0201, progress=True):
     path = os.path.join(sub, f"{H:06.4f}-res.dat")
     analysis.write_res(path, analysis.ladder(T), cols, 2 * n * 1024)
     if samples % 64 == 0 and samples >= 128:
-        jk = group_jackknife_errors(means, L)
-        np.savetxt(jk_path(path), np.column_stack([analysis.ladder(T), jk.T]), fmt="%16.12f",
-                   header="beta, sigma of chi0, xi_L/L, e, m: leave-one-lane-group-out jackknife")
+        analysis.write_jk(analysis.jk_path(path), analysis.ladder(T),
+                          analysis.group_jackknife_errors(means, L))
     ms, flips = sim.timing()
     return path, flips / (ms * 1e6)
-
-
-def jk_path(res_path):
-    return res_path[:-len("-res.dat")] + "-res.jk.dat"
-
-
-def group_jackknife_errors(means, L, block=64):
-    """Errors of chi0, xi_L/L, e, m ([4][T]) with the 64-sample lane groups
-    as the independent units.  The 64 samples of one uint64 word share the
-    random number of every site update, so their thermal fluctuations are
-    correlated and the table's own formula (variance over samples divided by
-    the sample count, analyze-all.py:44-68) understates the error where
-    thermal noise dominates (high temperature).  Leave-one-group-out
-    jackknife over the groups."""
-    n = means[0].shape[0]
-    G = n // block
-    th = []
-    for g in range(G):
-        keep = np.r_[0:g * block, (g + 1) * block:n]
-        c = analysis.estimators_from_time_means(*[a[keep] for a in means], L)
-        th.append([c[0], c[2], c[4], c[6]])
-    th = np.asarray(th)
-    return np.sqrt((G - 1) / G * ((th - th.mean(axis=0)) ** 2).sum(axis=0))
 
 
 def compare(ours, ref):
@@ -99,7 +75,8 @@ def compare(ours, ref):
     chi0, xi_L/L, e, m, with the table's own errors and, where the jackknife
     file exists, with the lane-group jackknife errors for ours (zb)."""
     a, b = np.loadtxt(ours), np.loadtxt(ref)
-    jk = np.loadtxt(jk_path(ours)) if os.path.exists(jk_path(ours)) else None
+    jkp = analysis.jk_path(ours)
+    jk = np.loadtxt(jkp) if os.path.exists(jkp) else None
     print(" beta      z(chi0)  z(xi/L)  z(e)    z(m)   |  zb(chi0) zb(xi/L) zb(e)   zb(m)")
     z = np.zeros((len(a), 4))
     zb = np.zeros((len(a), 4))

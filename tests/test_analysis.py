@@ -106,22 +106,63 @@ def test_group_jackknife_errors():
     """The lane-group jackknife agrees with the per-sample formula when
     samples are independent and picks up a fluctuation shared by the 64
     samples of a group, which the per-sample formula misses."""
-    import reproduce_tables
     rng = np.random.default_rng(7)
     S, T, L = 8192, 2, 8
     means = [rng.random((S, T)) + 1.0, rng.random((S, T, 3)) * 0.1 + 0.05,
              rng.random((S, T, 3)) * 0.1 + 0.05, -rng.random((S, T)), rng.random((S, T)) * 0.01]
-    jk = reproduce_tables.group_jackknife_errors(means, L)
+    jk = analysis.group_jackknife_errors(means, L)
     naive = analysis.estimators_from_time_means(*means, L)
     for k, col in enumerate((1, 3, 5, 7)):
         np.testing.assert_allclose(jk[k], naive[col], rtol=0.35)
     shared = np.repeat(rng.normal(0.0, 0.05, (S // 64, T)), 64, axis=0)
     means[3] = means[3] + shared
-    jk = reproduce_tables.group_jackknife_errors(means, L)
+    jk = analysis.group_jackknife_errors(means, L)
     naive = analysis.estimators_from_time_means(*means, L)
     expected = math.sqrt((1 / 12) / S + 0.05 ** 2 / (S // 64))
     np.testing.assert_allclose(jk[2], expected, rtol=0.35)
     assert np.all(jk[2] > 1.4 * naive[5])
+
+
+def test_group_jackknife_of_linear_columns_is_the_group_standard_error():
+    """For chi0, e and m (plain means over samples) the leave-one-group-out
+    jackknife equals the standard error of the group means,
+    sqrt(sum_g (x_g - x)^2 / (G (G - 1))): a closed form that a wrong
+    weight, a dropped group or an off-by-one in the blocks would miss."""
+    rng = np.random.default_rng(11)
+    S, T, L, B = 640, 3, 8, 64
+    means = [rng.random((S, T)) + 1.0, rng.random((S, T, 3)) * 0.1 + 0.05,
+             rng.random((S, T, 3)) * 0.1 + 0.05, -rng.random((S, T)), rng.random((S, T)) * 0.01]
+    jk = analysis.group_jackknife_errors(means, L, block=B)
+    G = S // B
+    for k, a in ((0, means[0]), (2, means[3]), (3, means[4])):
+        xg = a.reshape(G, B, T).mean(axis=1)
+        se = np.sqrt(((xg - xg.mean(axis=0)) ** 2).sum(axis=0) / (G * (G - 1)))
+        np.testing.assert_allclose(jk[k], se, rtol=1e-10)
+    assert np.all(np.isfinite(jk[1])) and np.all(jk[1] > 0)
+
+
+def test_analyze_obs_writes_the_jackknife_file(tmp_path):
+    """analysis.py --jackknife: the res.jk.dat beside res.dat holds, per
+    temperature, beta and the jackknife errors computed from the second half
+    of obs.bin."""
+    rng = np.random.default_rng(5)
+    L, S, T, n_rec = 8, 128, 4, 6
+    rec = np.zeros((n_rec, S, T), dtype=analysis.OBS_DTYPE)
+    for f in ("e", "m"):
+        rec[f] = rng.normal(0.0, 0.3, rec[f].shape)
+    for f in ("q_1_r", "q_1_i"):                     # chi(k) below chi(0): xi is defined
+        rec[f] = rng.normal(0.0, 0.1, rec[f].shape)
+    rec["q_0"] = rng.normal(0.0, 0.3, rec["q_0"].shape)
+    os.makedirs(tmp_path / "08")
+    rec.tofile(tmp_path / "08" / "0.0000-obs.bin")
+    out = analysis.analyze_obs(str(tmp_path), L, 0.0, S, T, jackknife=True)
+    jk = np.loadtxt(analysis.jk_path(out))
+    assert jk.shape == (T, 5)
+    np.testing.assert_allclose(jk[:, 0], analysis.ladder(T), rtol=1e-11)
+    means = analysis.time_means(rec[n_rec // 2:])
+    assert np.all(np.isfinite(jk)) and np.all(jk[:, 1:] > 0)
+    np.testing.assert_allclose(jk[:, 1:].T, analysis.group_jackknife_errors(means, L),
+                               rtol=1e-9, atol=1e-12)
 
 
 def test_plots_of_the_reproduced_tables(tmp_path):

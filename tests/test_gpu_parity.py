@@ -66,6 +66,25 @@ def test_sweeps_pt_measurements_match_oracle(L):
     np.testing.assert_array_equal(sim.results(), ref)
 
 
+@pytest.mark.parametrize("form", [sg.SG_FORM_PLAIN, sg.SG_FORM_BATCHED])
+@pytest.mark.parametrize("L", [12, 14])
+def test_config2_shape_at_L12_14_matches_oracle(L, form):
+    """The plain drop-in configuration at L = 12 and 14 with its full
+    ladder: 64 samples, 2 replicas, 24 temperatures geometric in T = [0.2, 2],
+    PT every 16 sweeps, a measurement every 48; three PT intervals in the
+    plain and the batched form (the tuner's shape).  Spins, energies, swap
+    counts, records and FP64 results equal the oracle's."""
+    betas = geometric_betas(0.2, 2.0, 24)
+    sim, orc = _pair(L, 64, 2, betas, pt_every=16, meas_every=48, form=form)
+    sim.run(48)
+    orc.run(48)
+    _assert_same(sim, orc)
+    assert len(sim.records) == orc.n_meas() == 1
+    np.testing.assert_array_equal(sim.records[0][1], orc.meas(0))
+    ref = results_from_records([orc.meas(0)], L, 2, 0.0, 64, betas)
+    np.testing.assert_array_equal(sim.results(), ref)
+
+
 @pytest.mark.parametrize("L", [18, 20, 22, 24, 26, 28, 32, 48])
 def test_large_lattice_sweeps_match_oracle(L):
     """L > 16 runs the resident large-lattice kernels: two lattices in LDS
