@@ -28,9 +28,9 @@ FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-ffp-contract=
 
 LIB_SOURCES = ["sg_kernels_resident.hip", "sg_kernels_split.hip", "sg_kernels.hip", "sg_runtime.cpp", "sg_engine.cpp",
                "sg_probe.hip", "sg_static.hip"]
-# The resident sweep kernels take LLVM's max-ILP machine scheduler (faster for
-# them; slower for the streamed kernels, which keep the default).
-SOURCE_FLAGS = {"sg_kernels_resident.hip": ["-mllvm", "--amdgpu-sched-strategy=max-ilp"]}
+# Per-source extra flags (none at present: every object takes the default
+# machine scheduler).
+SOURCE_FLAGS: dict[str, list[str]] = {}
 
 
 def _compile(src: str) -> str:
