@@ -12,11 +12,12 @@ over the whole batch):
   interval, and the drop-in shape itself (64 samples, one group).
 value = batched spin-flip attempts per ns (whole job, inputs resident in HBM).
 
-The same line carries the serving loop (Poisson arrivals at
-200 jobs/s and at a rate one GPU sustains, p50/p99 arrival-to-result latency),
+The same line carries the serving loop (jobs of mixed L arriving as a Poisson
+stream at 200 jobs/s and at a rate one GPU sustains, p50/p99 arrival-to-result
+latency),
 the VALU issue-rate probe behind the roofline's peak, and the CPU baseline
-(the C oracle on the host's cores: config 1 and the L=16 twin, one thread and
-all threads).
+(the C oracle on the host's cores: the reference's CPU case, L=8 with 32
+samples, and the L=16 workload, one thread and all threads).
 
 Other workloads (one JSON line each, same keys):
   mixed  8 groups of 64 samples, L per group drawn from {8,12,16,20,24}
@@ -27,7 +28,7 @@ Other workloads (one JSON line each, same keys):
   field  the default shape at H != 0 (--field-h).
 --valu-probe prints the probe's JSON alone (for rocprofv3 counter runs).
 
-Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+Prints one JSON line.  --gpus N with N > 1 exits: the engine runs on one GPU.
 """
 from __future__ import annotations
 
@@ -156,7 +157,7 @@ def _oracle_rate(oracle, Lc, samples, n_temps, t_lo, t_hi, pt_every, meas_every,
 def cpu_baseline():
     """The C oracle (the same bit-sliced algorithm and Philox numbers as the
     GPU, OpenMP over lattices with one PT synchronisation per step) on this
-    host's cores: config 1 (L=8, 32 samples, 16 T in [0.5, 2], PT every sweep,
+    host's cores: the reference's CPU case (L=8, 32 samples, 16 T in [0.5, 2], PT every sweep,
     a measurement every 100) and the L=16 twin of the default line (64
     samples, 24 T, PT every 16), each with one thread and with all the
     threads this process may use (OMP_NUM_THREADS when set, else the CPU
@@ -172,8 +173,8 @@ def cpu_baseline():
               meas_every=MEAS_EVERY)
     t0 = time.perf_counter()
     r = {
-        "config1_1t": _oracle_rate(oracle, **c1, sweeps=2000, warm=100, threads=1, repeats=3),
-        "config1_all": _oracle_rate(oracle, **c1, sweeps=10000, warm=100, threads=threads,
+        "l8_1t": _oracle_rate(oracle, **c1, sweeps=2000, warm=100, threads=1, repeats=3),
+        "l8_all": _oracle_rate(oracle, **c1, sweeps=10000, warm=100, threads=threads,
                                     repeats=3),
         "l16_1t": _oracle_rate(oracle, **c2, sweeps=256, warm=32, threads=1, repeats=3),
         "l16_all": _oracle_rate(oracle, **c2, sweeps=1024, warm=96, threads=threads, repeats=5),
@@ -183,14 +184,14 @@ def cpu_baseline():
         value=round(r["l16_all"], 3), unit="spin flips/ns", cores=threads, kind="port",
         cpu_model=_cpu_model(), host_cpus=host_cpus,
         single_thread=round(r["l16_1t"], 3),
-        config1={"single_thread": round(r["config1_1t"], 3), "all_threads": round(r["config1_all"], 3),
+        reference_cpu_case_L8={"single_thread": round(r["l8_1t"], 3), "all_threads": round(r["l8_all"], 3),
                  "workload": "L=8, 32 samples (half a 64-lane word; samples counted), 2 replicas, "
                              "16 T geometric in [0.5,2.0], PT every sweep, measurement every 100; "
                              "1e4 sweeps with all threads, 2e3 with one"},
         sample=f"oracle/sg_oracle.c bit-sliced path (gcc -O3 -march=x86-64-v3, OpenMP over "
                f"lattices): value = L=16, 64 samples x {T} T x {R} R, PT every {PT_EVERY}, "
                f"1024 sweeps after 96 untimed, {threads} threads, median of 5; single_thread = "
-               f"the same, 256 sweeps, median of 3; config1 as stated; {spent:.1f} s in all")
+               f"the same, 256 sweeps, median of 3; reference_cpu_case_L8 as stated; {spent:.1f} s in all")
 
 
 def copy_probe(mib):
@@ -235,7 +236,7 @@ def main():
     ap.add_argument("--valu-probe", action="store_true", help="print the VALU probe alone")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (samples shard across GPUs with no "
+        print("bench.py: the engine runs on one GPU only (samples shard across GPUs with no "
               "exchange); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.valu_probe:

@@ -68,7 +68,7 @@ def test_sweeps_pt_measurements_match_oracle(L):
 
 @pytest.mark.parametrize("form", [sg.SG_FORM_PLAIN, sg.SG_FORM_BATCHED])
 @pytest.mark.parametrize("L", [12, 14])
-def test_config2_shape_at_L12_14_matches_oracle(L, form):
+def test_plain_dropin_shape_at_L12_14_matches_oracle(L, form):
     """The plain drop-in configuration at L = 12 and 14 with its full
     ladder: 64 samples, 2 replicas, 24 temperatures geometric in T = [0.2, 2],
     PT every 16 sweeps, a measurement every 48; three PT intervals in the
@@ -154,7 +154,7 @@ def test_other_large_sizes_match_oracle(L, H, kernel):
          form=sg.SG_FORM_BATCHED, kernel=sg.SG_KERNEL_RESIDENT),      # captured intervals
     dict(L=32, n=64, R=1, T=(0.3, 2.0, 2), pt=2, meas=0,             # slab sweeps, one replica
          kernel=sg.SG_KERNEL_STREAMED),
-], ids=["config1_L8", "R1", "T1", "noPT", "L24_3groups_batched", "L32_slab_R1"])
+], ids=["reference_cpu_case_L8", "R1", "T1", "noPT", "L24_3groups_batched", "L32_slab_R1"])
 def test_edge_shapes_match_oracle(case):
     tmin, tmax, nt = case["T"]
     betas = geometric_betas(tmin, tmax, nt)
