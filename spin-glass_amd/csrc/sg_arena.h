@@ -23,7 +23,7 @@ class DevArena {
   DevArena(const DevArena &) = delete;
   DevArena &operator=(const DevArena &) = delete;
   ~DevArena() {
-    if (base_) hipFree(base_);
+    if (base_) (void)hipFree(base_);
   }
 
   hipError_t init(size_t bytes) {

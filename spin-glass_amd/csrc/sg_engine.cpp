@@ -33,7 +33,7 @@ class HostPool {
   HostPool(const HostPool &) = delete;
   HostPool &operator=(const HostPool &) = delete;
   ~HostPool() {
-    for (auto &b : free_) hipHostFree(b.second);
+    for (auto &b : free_) (void)hipHostFree(b.second);
   }
   static size_t size_class(size_t n) {
     size_t c = 4096;
@@ -87,7 +87,7 @@ struct Job {
       pool->give(h_acc, acc_bytes);
       pool->give(h_cnt, cnt_bytes);
     }
-    if (done_ev) hipEventDestroy(done_ev);
+    if (done_ev) (void)hipEventDestroy(done_ev);
     h_acc = nullptr;
     h_cnt = nullptr;
     done_ev = nullptr;
@@ -106,8 +106,8 @@ struct Grave {
   }
   void free_all() {
     for (auto &a : arrays) {
-      if (a.first) hipFree(a.first);
-      if (a.second) hipHostFree(a.second);
+      if (a.first) (void)hipFree(a.first);
+      if (a.second) (void)hipHostFree(a.second);
     }
     arrays.clear();
   }
@@ -119,8 +119,8 @@ struct DevList {
   size_t cap = 0;
   int n = 0;
   void free_all() {
-    hipFree(d);
-    if (h) hipHostFree(h);
+    (void)hipFree(d);
+    if (h) (void)hipHostFree(h);
     d = h = nullptr;
     cap = 0;
   }
@@ -176,15 +176,15 @@ struct Class {
         ++i;
         continue;
       }
-      hipEventSynchronize(old[i].first);
-      hipEventDestroy(old[i].first);
-      for (auto &g : old[i].second) hipGraphExecDestroy(g.exec);
+      (void)hipEventSynchronize(old[i].first);
+      (void)hipEventDestroy(old[i].first);
+      for (auto &g : old[i].second) (void)hipGraphExecDestroy(g.exec);
       old.erase(old.begin() + (ptrdiff_t)i);
     }
   }
 
   void free_all() {
-    if (st) hipStreamSynchronize(st);
+    if (st) (void)hipStreamSynchronize(st);
     retire_graphs();
     reap_graphs(true);
     grave.free_all();
@@ -195,8 +195,8 @@ struct Class {
       tasks[k].free_all();
       ptc[k].free_all();
     }
-    if (up_ev) hipEventDestroy(up_ev);
-    if (st) hipStreamDestroy(st);
+    if (up_ev) (void)hipEventDestroy(up_ev);
+    if (st) (void)hipStreamDestroy(st);
   }
 };
 
@@ -219,13 +219,13 @@ struct sg_batch {
 
   ~sg_batch() {
     for (auto &s : inflight)
-      for (auto e : s) hipEventSynchronize(e);
+      for (auto e : s) (void)hipEventSynchronize(e);
     for (auto &c : classes)
-      if (c.st) hipStreamSynchronize(c.st);
+      if (c.st) (void)hipStreamSynchronize(c.st);
     jobs.clear();  // job handles free into the arena before it goes
     for (auto &s : inflight)
-      for (auto e : s) hipEventDestroy(e);
-    for (auto e : ev_pool) hipEventDestroy(e);
+      for (auto e : s) (void)hipEventDestroy(e);
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
     for (auto &c : classes) c.free_all();
   }
 
@@ -586,12 +586,12 @@ struct sg_batch {
           launch_class_interval(c, k ^ (i & 1), i + 1 < count ? e_mid : e_last);
       } catch (...) {
         // leave the stream out of capture mode before reporting
-        if (hipStreamEndCapture(c.st, &g) == hipSuccess && g) hipGraphDestroy(g);
+        if (hipStreamEndCapture(c.st, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
         throw;
       }
       HIPCHK(hipStreamEndCapture(c.st, &g));
       const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
+      (void)hipGraphDestroy(g);
       HIPCHK(e);
       c.graphs.push_back(Class::Graph{k, count, e_last, exec});
       ++graphs_built;

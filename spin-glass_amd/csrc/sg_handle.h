@@ -250,7 +250,7 @@ struct sg_handle {
       if (arena)
         arena->release((void *)p);
       else
-        hipFree((void *)p);
+        (void)hipFree((void *)p);
     }
     p = nullptr;
   }
@@ -386,21 +386,21 @@ struct sg_handle {
     dev_free(d_self); dev_free(d_Js); dev_free(d_chunks); dev_free(d_ptc[0]); dev_free(d_ptc[1]);
     dev_free(d_spins2); dev_free(d_slabs);
     dev_free(d_split); dev_free(d_xbuf); dev_free(d_xflag); dev_free(d_xerr); dev_free(d_ticket);
-    for (auto &g : graphs) hipGraphExecDestroy(g.exec);
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();
     for (int i = 0; i < kRing; ++i) {
-      if (h_ring[i]) hipHostFree(h_ring[i]);
-      if (ring_ev[i]) hipEventDestroy(ring_ev[i]);
+      if (h_ring[i]) (void)hipHostFree(h_ring[i]);
+      if (ring_ev[i]) (void)hipEventDestroy(ring_ev[i]);
       h_ring[i] = nullptr;
       ring_ev[i] = nullptr;
     }
-    for (auto &p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     pending.clear();
-    for (auto e : ev_pool) hipEventDestroy(e);
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
     ev_pool.clear();
-    if (ev0) hipEventDestroy(ev0);
-    if (ev1) hipEventDestroy(ev1);
-    if (st && own_stream) hipStreamDestroy(st);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (st && own_stream) (void)hipStreamDestroy(st);
     ev0 = ev1 = nullptr; st = nullptr;
   }
 
@@ -631,7 +631,7 @@ struct sg_handle {
     streamed = s;
     single_shape = one;
     split_shape = sp;
-    for (auto &g : graphs) hipGraphExecDestroy(g.exec);
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();
   }
 
@@ -966,12 +966,12 @@ struct sg_handle {
       for (int i = 0; i < count; ++i) launch_interval((par + i) & 1, 1, seg, want_energy);
     } catch (...) {
       // leave the stream out of capture mode before reporting
-      if (hipStreamEndCapture(st, &g) == hipSuccess && g) hipGraphDestroy(g);
+      if (hipStreamEndCapture(st, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
       throw;
     }
     HIPCHK(hipStreamEndCapture(st, &g));
     hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     HIPCHK(e);
     graphs.push_back(GraphKey{par, seg, want_energy, count, exec});
     return exec;

@@ -125,10 +125,10 @@ sg_status sg_probe_valu(int32_t device, int32_t kind, int32_t iters, double *lan
       break;
     }
   } while (false);
-  if (a) hipEventDestroy(a);
-  if (b) hipEventDestroy(b);
-  if (st) hipStreamDestroy(st);
-  hipFree(out);
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(out);
   if (rc != SG_OK) return rc;
   *lane_ops = (double)kProbeThreads * kProbeBlocks * (double)iters * kChains;
   *ms = t;
