@@ -2420,6 +2420,268 @@ __global__ __launch_bounds__((SplitGeo<L, P>::NT)) void k_split(const JobPtrs *_
   }
 }
 
+// ------------------------------------------------- wide split sweep kernel
+// The split sweep for lattices whose slab pair does not fit in LDS with both
+// colours (instantiated for L = 48, P = 4: D = 6 planes per half, 147 KB per
+// colour copy).  Ownership, Philox quads, the exchange protocol, tickets,
+// flags and energies are those of k_split; what differs:
+//  * LDS holds one colour at a time: during the phase of colour c, the copy
+//    of colour 1 - c (own planes plus a halo plane below and above each
+//    half), which is all the phase reads.  The new colour-c words stay in
+//    VGPRs until every wave has finished reading, then replace the copy,
+//    and the neighbours' boundary planes of colour c fill its halo rows.
+//  * thread t owns QPT = D L^2 / 2 NT quads q = t + m NT (m = 0 .. QPT-1),
+//    numbered through the planes in the order 0, D - 1, 1, 2, ..., D - 2,
+//    so that quads m < NB = L^2 / NT lie on the two boundary planes: every
+//    thread updates NB boundary quads, publishes them, then its inner quads.
+//    Each quad's (column, plane) is packed in one VGPR.
+// Reference counterpart: perform_MC_steps, Program/src/eamsim.cu:112-208,
+// whose lattice stays in one block's shared memory (L <= 18 there).
+template <int L, int P, int NT>
+struct WideGeo {
+  static constexpr int N = L * L * L;
+  static constexpr int NH = N / 2;
+  static constexpr int LH = L / 2;
+  static constexpr int PL = LH * L;             // columns: words per plane of one colour
+  static constexpr int D = L / (2 * P);         // planes per half slab
+  static constexpr int QW = D * PL;             // quads per slab pair
+  static constexpr int QPT = QW / NT;           // quads per thread
+  static constexpr int NB = 2 * PL / NT;        // of them on the boundary planes: m < NB
+  static constexpr int ROWS = D + 2;            // halo, D own planes, halo
+  static constexpr int CW = 2 * ROWS * PL;      // the colour copy [half][row][column]
+  static constexpr int PU = ceil_log2(12 * QPT + 1);
+  static constexpr int PK = ceil_log2(4 * QPT + 1);
+  static_assert(P >= 2 && L % (2 * P) == 0 && D >= 2, "slab pairs with two boundary planes");
+  static_assert(NT % 64 == 0 && NT <= 1024 && QW % NT == 0 && (2 * PL) % NT == 0,
+                "every thread owns the same number of boundary and inner quads");
+  static_assert(CW * 8 + 4 * 64 * 4 + 4 <= 160 * 1024, "the colour copy in LDS");
+  static_assert(PU * NT <= CW, "fold scratch in the copy");
+};
+
+template <int L, int P, int NT, bool FIELD>
+__global__ __launch_bounds__(NT) void k_wide(const JobPtrs *__restrict__ jt,
+                                             const TaskDesc *__restrict__ tasks,
+                                             uint32_t *__restrict__ ticket, int nsweeps,
+                                             int want_energy) {
+  using G = WideGeo<L, P, NT>;
+  __shared__ uint64_t cb[G::CW];            // [half][row][column], one colour
+  __shared__ int32_t redi[4 * 64];
+  __shared__ uint32_t s_ticket;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_ticket = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t slot = __builtin_amdgcn_readfirstlane(s_ticket) % gridDim.x;
+  const int ti = (int)(slot / P), w = (int)(slot % P);
+  const TaskDesc tk = ld_uniform(tasks + ti);
+  const JobPtrs jp = ld_uniform(jt + tk.job);
+  const LatDesc d = ld_uniform(jp.descs + tk.lat[0]);
+  SG_GLOBAL uint64_t *__restrict__ g = to_global(jp.spins) + d.spin_off;
+  SG_GLOBAL int32_t *__restrict__ energy = to_global(jp.energy) + (size_t)d.e_idx * 128;
+  const SG_GLOBAL char *__restrict__ Js = (const SG_GLOBAL char *)(to_global(jp.Js) + 2 * d.j_off);
+  uint64_t *xb = jp.xbuf + (size_t)tk.lat[0] * P * 8 * G::PL;
+  uint32_t *xf = jp.xflag + (size_t)tk.lat[0] * P;
+  const int wb = (w + P - 1) % P, wa = (w + 1) % P;   // neighbours below, above
+  const int zw = w * G::D;                             // first plane of the lower half
+
+  // quad m: q = tid + m NT; plane index pi = q / PL in the order 0, D-1, 1, ...
+  uint32_t qd[G::QPT];                                 // column | plane << 16
+#pragma unroll
+  for (int m = 0; m < G::QPT; ++m) {
+    const int q = tid + m * NT, pi = q / G::PL;
+    const int zl = pi == 0 ? 0 : pi == 1 ? G::D - 1 : pi - 1;
+    qd[m] = (uint32_t)(q % G::PL) | ((uint32_t)zl << 16);
+  }
+  // read through pinned(): each use recomputes its addresses, so none is
+  // held across the sweep loop
+  auto col_of = [&](int m) { return pinned((int)qd[m]) & 0xFFFF; };
+  auto zl_of = [&](int m) { return pinned((int)qd[m]) >> 16; };
+  auto row_of = [&](int m, int hs) { return (hs * G::ROWS + zl_of(m) + 1) * G::PL; };
+  auto gidx = [&](int m, int hs, int c) {
+    return c * G::NH + col_of(m) + G::PL * (zw + zl_of(m) + hs * G::LH);
+  };
+
+  // the registers hold the colour the phase updates (colour 0 first), the
+  // copy the other one: own planes and halos, from the lattice
+  uint64_t sp[G::QPT][2];                              // [quad][half]
+#pragma unroll
+  for (int m = 0; m < G::QPT; ++m)
+#pragma unroll
+    for (int hs = 0; hs < 2; ++hs) {
+      sp[m][hs] = ld_word(g, gidx(m, hs, 0));
+      cb[row_of(m, hs) + col_of(m)] = ld_word(g, gidx(m, hs, 1));
+    }
+  for (int i = tid; i < 4 * G::PL; i += NT) {
+    const int hs = i / (2 * G::PL), top = (i / G::PL) & 1, cl = i % G::PL;
+    const int z = (zw + hs * G::LH + (top ? G::D : -1) + L) % L;
+    cb[(hs * G::ROWS + (top ? G::D + 1 : 0)) * G::PL + cl] = ld_word(g, G::NH + cl + G::PL * z);
+  }
+  ThrRec tr = ld_uniform(jp.thr + d.thr_idx);
+  const uint32_t n0 = ld_uniform(jp.js).sweep;
+  if (want_energy && w == 0 && tid < (want_energy > 1 ? 128 : 64))
+    atomicExch((int32_t *)(energy + tid), 0);
+  __syncthreads();
+
+  // the six neighbours of own site (m, hs) at colour c, all from the copy of
+  // colour 1 - c: +x, -x, +y, -y, +z, -z
+  auto neighbours = [&](int m, int hs, int c, uint64_t (&nv)[6]) {
+    const int col = col_of(m), xh = col % G::LH, y = col / G::LH;
+    const int p = (y + zl_of(m) + zw + hs * G::LH + c) & 1;   // x = 2 xh + p
+    const int row = row_of(m, hs);
+    const int xo = p ? (xh == G::LH - 1 ? -(G::LH - 1) : 1) : (xh == 0 ? G::LH - 1 : -1);
+    const uint64_t xl = cb[row + col + xo], xr = cb[row + col];
+    nv[0] = p ? xl : xr;
+    nv[1] = p ? xr : xl;
+    nv[2] = cb[row + (y == L - 1 ? col - (L - 1) * G::LH : col + G::LH)];
+    nv[3] = cb[row + (y == 0 ? col + (L - 1) * G::LH : col - G::LH)];
+    nv[4] = cb[row + G::PL + col];
+    nv[5] = cb[row - G::PL + col];
+  };
+  auto record = [&](int m, int hs, int c, uint64_t (&jw)[6]) {
+    site_record(Js, gidx(m, hs, c), true, jw);
+  };
+
+  const uint32_t sw = (kTagMC << 28) | (d.rt & 0x0FFFFFFFu);
+  uint32_t r23[G::QPT][2];
+  for (int it = 0; it < nsweeps; ++it) {
+    const uint32_t n = n0 + (uint32_t)it;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const uint32_t phase = 2u * n + (uint32_t)c;
+      uint64_t *xme = xb + ((size_t)w * 2 + (phase & 1)) * 4 * G::PL;   // [hs][side][PL]
+      auto quad = [&](int m) {
+        uint32_t rw[2];
+        if (c == 0) {
+          const uint32_t j = (uint32_t)(col_of(m) + G::PL * (zw + zl_of(m)));
+          const u32x4 r = philox_sweep(j, n, d.g, sw, d.key0, d.key1);
+          rw[0] = r.v[0];
+          rw[1] = r.v[1];
+          r23[m][0] = r.v[2];
+          r23[m][1] = r.v[3];
+        } else {
+          rw[0] = r23[m][0];
+          rw[1] = r23[m][1];
+        }
+        uint64_t jc[6], jn[6];
+        record(m, 0, c, jc);
+#pragma unroll
+        for (int hs = 0; hs < 2; ++hs) {
+          if (hs == 0) record(m, 1, c, jn);
+          uint64_t nv[6];
+          neighbours(m, hs, c, nv);
+          update_site<L, FIELD>(sp[m][hs], nv, jc, rw[hs], tr);
+#pragma unroll
+          for (int e = 0; e < 6; ++e) jc[e] = jn[e];
+          if (m < G::NB)
+            st_coherent(xme, (hs * 2 + (zl_of(m) == 0 ? 0 : 1)) * G::PL + col_of(m), sp[m][hs]);
+        }
+      };
+      // boundary quads, their planes out, the phase published
+#pragma unroll
+      for (int m = 0; m < G::NB; ++m) {
+        quad(m);
+      }
+      __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this thread's planes are out
+      __syncthreads();
+      if (tid == 0)
+        __hip_atomic_store(xf + w, phase + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // inner quads while the planes travel
+#pragma unroll
+      for (int m = G::NB; m < G::QPT; ++m) {
+        quad(m);
+      }
+      __syncthreads();                      // every read of the colour 1 - c copy is done
+      // the new colour-c words into the copy, the colour 1 - c words (the
+      // next phase's) out of it
+#pragma unroll
+      for (int m = 0; m < G::QPT; ++m)
+#pragma unroll
+        for (int hs = 0; hs < 2; ++hs) {
+          const int pos = row_of(m, hs) + col_of(m);
+          const uint64_t o = cb[pos];
+          cb[pos] = sp[m][hs];
+          sp[m][hs] = o;
+        }
+      if (tid == 0) {
+        uint32_t polls = 0;
+        for (;;) {
+          const uint32_t fb = __hip_atomic_load(xf + wb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const uint32_t fa = __hip_atomic_load(xf + wa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if ((int32_t)(fb - (phase + 1u)) >= 0 && (int32_t)(fa - (phase + 1u)) >= 0) break;
+          if (++polls > kSplitSpin) {
+            atomicOr(jp.xerr, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      __syncthreads();
+      // halo rows of colour c: below <- the top planes of slab pair wb, above
+      // <- the bottom planes of wa; across the wrap the halves trade places
+      const uint64_t *xbl = xb + ((size_t)wb * 2 + (phase & 1)) * 4 * G::PL;
+      const uint64_t *xab = xb + ((size_t)wa * 2 + (phase & 1)) * 4 * G::PL;
+      for (int i = tid; i < 4 * G::PL; i += NT) {
+        const int hs = i / (2 * G::PL), top = (i / G::PL) & 1, cl = i % G::PL;
+        const uint64_t v =
+            top ? ld_coherent(xab, ((hs ^ (w == P - 1 ? 1 : 0)) * 2 + 0) * G::PL + cl)
+                : ld_coherent(xbl, ((hs ^ (w == 0 ? 1 : 0)) * 2 + 1) * G::PL + cl);
+        cb[(hs * G::ROWS + (top ? G::D + 1 : 0)) * G::PL + cl] = v;
+      }
+      __syncthreads();
+    }
+  }
+
+  // after an even number of phases the registers hold colour 0 and the copy
+  // colour 1 with its halos: U over the colour-0 ends, K over both colours
+  uint64_t AU[1][G::PU] = {}, AK[1][G::PK] = {};
+  if (want_energy) {
+#pragma unroll
+    for (int m = 0; m < G::QPT; ++m)
+#pragma unroll
+      for (int hs = 0; hs < 2; ++hs) {
+        uint64_t jc[6], nv[6];
+        record(m, hs, 0, jc);
+        neighbours(m, hs, 0, nv);
+        uint64_t u0, u1, u2;
+        bond_count(sp[m][hs], nv, jc, u0, u1, u2);
+        vc_add3(AU[0], u0, u1, u2);
+        __builtin_amdgcn_sched_barrier(0);   // one site's record live at a time
+      }
+  }
+  if (want_energy > 1) {
+#pragma unroll
+    for (int m = 0; m < G::QPT; ++m)
+#pragma unroll
+      for (int hs = 0; hs < 2; ++hs) {
+        vc_add1(AK[0], sp[m][hs]);
+        vc_add1(AK[0], cb[row_of(m, hs) + col_of(m)]);
+      }
+  }
+#pragma unroll
+  for (int m = 0; m < G::QPT; ++m)
+#pragma unroll
+    for (int hs = 0; hs < 2; ++hs) {
+      st_word(g, gidx(m, hs, 0), sp[m][hs]);
+      st_word(g, gidx(m, hs, 1), cb[row_of(m, hs) + col_of(m)]);
+    }
+  if (want_energy) {
+    __syncthreads();   // the copy serves as the fold scratch
+    int32_t U[1];
+    wg_count_lanes<G::PU, NT, 1>(AU, cb, redi, U);
+    if (tid < 64) atomicAdd((int32_t *)(energy + tid), U[0]);
+    if (want_energy > 1) {
+      __syncthreads();
+      int32_t Kc[1];
+      wg_count_lanes<G::PK, NT, 1>(AK, cb, redi, Kc);
+      if (tid < 64) atomicAdd((int32_t *)(energy + 64 + tid), Kc[0]);
+    }
+  }
+  // the last work-group out resets the ticket counter for the next launch
+  if (tid == 0 && atomicAdd(ticket + 1, 1u) == gridDim.x - 1) {
+    atomicExch(ticket, 0u);
+    atomicExch(ticket + 1, 0u);
+  }
+}
+
 // ---------------------------------------------------------------- dispatch
 #define SG_LDS_SIZES(X) X(2) X(4) X(6) X(8) X(10) X(12) X(14) X(16)
 #define SG_ALL_SIZES(X) SG_LDS_SIZES(X) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) \
@@ -2668,9 +2930,9 @@ hipError_t launch_advance(const JobPtrs *jt, int njobs, int nsweeps, hipStream_t
 #ifdef SG_SPLIT_PART
 // The split kernels, in a translation unit of their own (sg_kernels_split.hip).
 int split_slabs(int L, int n_lattices) {
-  // L = 32: four slab pairs of 4 + 4 planes per lattice, one work-group per
-  // CU (1024 threads, two quads each, 114 VGPRs), up to 64 lattices
-  return L == 32 && n_lattices * 4 <= 256 ? 4 : 0;
+  // L = 32: four slab pairs of 4 + 4 planes per lattice (k_split), L = 48:
+  // four of 6 + 6 planes (k_wide); one work-group per CU, up to 64 lattices
+  return (L == 32 || L == 48) && n_lattices * 4 <= 256 ? 4 : 0;
 }
 
 hipError_t launch_split(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int nlat,
@@ -2690,6 +2952,18 @@ hipError_t launch_split(int L, bool field, const JobPtrs *jt, const TaskDesc *ta
   }
   SG_SPLIT(32, 4)
 #undef SG_SPLIT
+#define SG_WIDE(l, p, nt)                                                                  \
+  if (L == l && P == p) {                                                                 \
+    if (field)                                                                             \
+      hipLaunchKernelGGL((k_wide<l, p, nt, true>), grid, dim3(nt), 0, st, jt, tasks, ticket,    \
+                         nsweeps, want_energy);                                            \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_wide<l, p, nt, false>), grid, dim3(nt), 0, st, jt, tasks, ticket,   \
+                         nsweeps, want_energy);                                            \
+    return hipGetLastError();                                                              \
+  }
+  SG_WIDE(48, 4, 768)
+#undef SG_WIDE
   return hipErrorInvalidValue;
 }
 #endif  // SG_SPLIT_PART

@@ -108,13 +108,15 @@ def test_large_lattice_sweeps_match_oracle(L):
 
 @pytest.mark.parametrize("form", [sg.SG_FORM_PLAIN, sg.SG_FORM_BATCHED])
 @pytest.mark.parametrize("H", [0.0, 0.5])
-def test_split_kernel_matches_oracle(H, form):
-    """L = 32 held by four work-groups per lattice (slab pairs exchanging
-    boundary planes every half-sweep): 80 samples (a full and a partial
-    group; the plain form launches each group's 16 lattices alone), PT every
-    4 sweeps, a measurement every 8, runs ending inside an interval."""
+@pytest.mark.parametrize("L", [32, 48])
+def test_split_kernel_matches_oracle(L, H, form):
+    """A lattice held by four work-groups (slab pairs exchanging boundary
+    planes every half-sweep; both colours in LDS at L = 32, one colour at a
+    time at L = 48): 80 samples (a full and a partial group; the plain form
+    launches each group's 16 lattices alone), PT every 4 sweeps, a
+    measurement every 8, runs ending inside an interval."""
     betas = geometric_betas(0.3, 2.0, 8)
-    sim, orc = _pair(32, 80, 2, betas, H=H, pt_every=4, meas_every=8, form=form,
+    sim, orc = _pair(L, 80, 2, betas, H=H, pt_every=4, meas_every=8, form=form,
                      kernel=sg.SG_KERNEL_SPLIT)
     for n in (4, 9, 11):
         sim.run(n)
