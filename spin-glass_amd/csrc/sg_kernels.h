@@ -64,7 +64,7 @@ bool slab_kernel_supported(int L);
 hipError_t launch_slab(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
                        int dir, int it, int mode, hipStream_t st);
 
-// ---- split sweeps: a lattice (L = 32) held on chip by P work-groups
+// ---- split sweeps: a lattice (L = 32, 48) held on chip by P work-groups
 // for a whole PT interval, exchanging boundary planes through jt's xbuf /
 // xflag (k_split).  split_slabs: the P used for n_lattices lattices of size
 // L (0: no split kernel for this L); P > 1 needs all n_lattices * P

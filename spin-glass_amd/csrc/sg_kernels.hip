@@ -2118,21 +2118,23 @@ __global__ __launch_bounds__(256) void k_pt_chunk(const JobPtrs *__restrict__ jt
 
 // ------------------------------------------------------------- split kernel
 // A lattice too large for one CU's LDS held on chip by P work-groups for a
-// whole PT interval (instantiated for L = 32, P = 4).  Work-group w of a
-// lattice owns the slab pair z in [wD, wD + D) and [wD + L/2, wD + D + L/2),
-// D = L / 2P, so each Philox quad it owns has all four of its sites there.
-// Its sites of both colours stay in VGPRs: thread t owns the quads
-// qi = t + k NT (k = 0, 1) of the slab pair, one on a boundary plane of the
-// halves (zl = 0 or D - 1) and one inside.  LDS holds a copy of each colour:
-// the slab pair's own planes plus a halo plane below and above each half, so
-// an update reads five neighbours from LDS and the sixth (the other colour's
-// word of the same index, same quad) from a register.
+// whole PT interval (instantiated for L = 32 and 48, P = 4).  Work-group w
+// of a lattice owns the slab pair z in [wD, wD + D) and [wD + L/2, wD + D +
+// L/2), D = L / 2P, so each Philox quad it owns has all four of its sites
+// there.  Thread t owns QPT = D L^2 / 2 NT quads q = t + m NT (m = 0 ..
+// QPT-1), numbered through the planes in the order 0, D - 1, 1, 2, ..., D -
+// 2, so that quads m < NB = L^2 / NT lie on the two boundary planes of the
+// halves.  Each quad's column and plane are packed in one VGPR.
 //
-// A phase (one colour c) updates each thread's boundary site first, writes
-// the new colour-c words into the LDS copy of colour c and the boundary
-// planes into the exchange buffer, and publishes the phase in its flag; it
-// then updates the inner sites while the planes travel, waits for its two
-// neighbours' flags and copies their planes into the halo rows of colour c.
+// A phase updates one colour c.  Its words stay in VGPRs; LDS holds a copy
+// of the other colour: the slab pair's own planes plus a halo plane below
+// and above each half, which is all the phase reads (six neighbours per
+// site).  Each thread updates its boundary quads first and writes them into
+// the exchange buffer, and the work-group publishes the phase in its flag;
+// the inner quads are updated while the planes travel.  Once every wave has
+// finished reading, the new colour-c words replace the copy (whose colour
+// 1 - c words go back to the VGPRs for the next phase), the work-group waits
+// for its two neighbours' flags and copies their planes into the halo rows.
 // Exchange data and flags go through device-coherent accesses (relaxed
 // agent-scope atomics, which pass the per-XCD L2s to the memory side); each
 // thread waits for its plane stores to complete before the barrier that
@@ -2152,29 +2154,13 @@ __global__ __launch_bounds__(256) void k_pt_chunk(const JobPtrs *__restrict__ jt
 // the host turns into an error.
 //
 // PT runs before the launch (k_pt_chunk); the launch sweeps nsweeps >= 1
-// times and, with want_energy, counts U (colour-0 ends, in registers) and K
-// per work-group; with P > 1 the slab pair w = 0 zeroes the lattice's energy
-// slots before its first flag and every work-group adds its counts at the
-// end, after its last exchange, which orders the adds after the zeroing.
+// times and, with want_energy, counts U (colour-0 ends) and K per
+// work-group; the slab pair w = 0 zeroes the lattice's energy slots before
+// its first flag and every work-group adds its counts at the end, after its
+// last exchange, which orders the adds after the zeroing.
+// Reference counterpart: perform_MC_steps, Program/src/eamsim.cu:112-208,
+// whose lattice stays in one block's shared memory (L <= 18 there).
 constexpr uint32_t kSplitSpin = 1u << 19;
-
-template <int L, int P>
-struct SplitGeo {
-  static constexpr int N = L * L * L;
-  static constexpr int NH = N / 2;
-  static constexpr int LH = L / 2;
-  static constexpr int PL = LH * L;             // words per plane of one colour
-  static constexpr int D = L / (2 * P);         // planes per half slab
-  static constexpr int QW = D * PL;             // quads per slab pair
-  static constexpr int NT = QW / 2;             // two quads per thread
-  static constexpr int ROWS = D + 2;            // halo, D own planes, halo
-  static constexpr int XW = 8 * PL;             // exchange words per (lattice, slab pair)
-  static constexpr int PU = ceil_log2(12 * 2 + 1);
-  static constexpr int PK = ceil_log2(4 * 2 + 1);
-  static_assert(L % (2 * P) == 0 && D == 4 && NT == 2 * PL && NT <= 1024,
-                "threads t < PL own planes 0 and 2, the others 1 and 3");
-  static_assert(2 * 2 * ROWS * PL * 8 <= 150 * 1024, "LDS copies of both colours");
-};
 
 // A value the compiler must recompute where it is used: keeps per-quad
 // addresses out of the registers held across the sweep loop.
@@ -2200,245 +2186,8 @@ __device__ __forceinline__ uint64_t ld_coherent(const uint64_t *base, int idx) {
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int L, int P, bool FIELD>
-__global__ __launch_bounds__((SplitGeo<L, P>::NT)) void k_split(const JobPtrs *__restrict__ jt,
-                                                              const TaskDesc *__restrict__ tasks,
-                                                              uint32_t *__restrict__ ticket,
-                                                              int nsweeps, int want_energy) {
-  using G = SplitGeo<L, P>;
-  constexpr int CW = 2 * G::ROWS * G::PL;   // words of one colour's copy
-  static_assert(((G::PU + 1) / 2) * G::NT <= 2 * CW, "fold scratch in the copies");
-  __shared__ uint64_t cb[2 * CW];           // [colour][half][row][column]
-  __shared__ int32_t redi[4 * 64];
-  __shared__ uint32_t s_ticket;
-  const int tid = threadIdx.x;
-  if (tid == 0) s_ticket = atomicAdd(ticket, 1u);
-  __syncthreads();
-  const uint32_t slot = __builtin_amdgcn_readfirstlane(s_ticket) % gridDim.x;
-  const int ti = (int)(slot / P), w = (int)(slot % P);
-  const TaskDesc tk = ld_uniform(tasks + ti);
-  const JobPtrs jp = ld_uniform(jt + tk.job);
-  const LatDesc d = ld_uniform(jp.descs + tk.lat[0]);
-  SG_GLOBAL uint64_t *__restrict__ g = to_global(jp.spins) + d.spin_off;
-  SG_GLOBAL int32_t *__restrict__ energy = to_global(jp.energy) + (size_t)d.e_idx * 128;
-  const SG_GLOBAL char *__restrict__ Js = (const SG_GLOBAL char *)(to_global(jp.Js) + 2 * d.j_off);
-  uint64_t *xb = jp.xbuf + (size_t)tk.lat[0] * P * G::XW;
-  uint32_t *xf = jp.xflag + (size_t)tk.lat[0] * P;
-  const int wb = (w + P - 1) % P, wa = (w + 1) % P;   // neighbours below, above
-  const int zw = w * G::D;                             // first plane of the lower half
-  // the boundary quad first: threads t < PL own planes 0 (k = 0, boundary)
-  // and 2; the others planes 1 and 3 (k = 1, boundary)
-  const bool upper = __builtin_amdgcn_readfirstlane(tid) >= G::PL;
-
-  // quad k: column col = t mod PL (xh + LH y), plane zl = t / PL + 2 k; site
-  // (hs, c) at z = zw + zl + hs L/2, checkerboard index col + PL z
-  const int col = tid % G::PL;
-  uint64_t sp[2][2][2];
-#pragma unroll
-  for (int k = 0; k < 2; ++k)
-#pragma unroll
-    for (int hs = 0; hs < 2; ++hs)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const int zl = tid / G::PL + 2 * k;
-        sp[k][hs][c] = ld_word(g, c * G::NH + col + G::PL * (zw + zl + hs * G::LH));
-        cb[c * CW + (hs * G::ROWS + zl + 1) * G::PL + col] = sp[k][hs][c];
-      }
-  // phase 0 reads colour 1: halo planes from the lattice
-  for (int i = tid; i < 4 * G::PL; i += G::NT) {
-    const int hs = i / (2 * G::PL), top = (i / G::PL) & 1, cl = i % G::PL;
-    const int z = (zw + hs * G::LH + (top ? G::D : -1) + L) % L;
-    cb[CW + (hs * G::ROWS + (top ? G::D + 1 : 0)) * G::PL + cl] = ld_word(g, G::NH + cl + G::PL * z);
-  }
-  ThrRec tr = ld_uniform(jp.thr + d.thr_idx);
-  const uint32_t n0 = ld_uniform(jp.js).sweep;
-  if (P > 1 && want_energy && w == 0 && tid < (want_energy > 1 ? 128 : 64))
-    atomicExch((int32_t *)(energy + tid), 0);
-  __syncthreads();
-
-  const int xh = col % G::LH, y = col / G::LH;
-  // the six neighbours of own site (k, hs) at colour c, from the copy of
-  // colour 1 - c: +x, -x, +y, -y, +z, -z (the x neighbour of the same index is
-  // the other colour's register word)
-  auto neighbours = [&](int k, int hs, int c, uint64_t (&nv)[6]) {
-    const int zl = pinned(tid) / G::PL + 2 * k;
-    const int p = (y + zl + zw + hs * G::LH + c) & 1;   // x = 2 xh + p
-    const int row = (1 - c) * CW + (hs * G::ROWS + zl + 1) * G::PL;
-    const int xo = p ? (xh == G::LH - 1 ? -(G::LH - 1) : 1) : (xh == 0 ? G::LH - 1 : -1);
-    const uint64_t xl = cb[row + col + xo], xr = sp[k][hs][1 - c];
-    nv[0] = p ? xl : xr;
-    nv[1] = p ? xr : xl;
-    nv[2] = cb[row + (y == L - 1 ? col - (L - 1) * G::LH : col + G::LH)];
-    nv[3] = cb[row + (y == 0 ? col + (L - 1) * G::LH : col - G::LH)];
-    nv[4] = cb[row + G::PL + col];
-    nv[5] = cb[row - G::PL + col];
-  };
-  auto record = [&](int k, int hs, int c, uint64_t (&jw)[6]) {
-    const int zl = pinned(tid) / G::PL + 2 * k;
-    site_record(Js, c * G::NH + col + G::PL * (zw + zl + hs * G::LH), true, jw);
-  };
-
-  const uint32_t sw = (kTagMC << 28) | (d.rt & 0x0FFFFFFFu);
-  uint32_t r23[2][2];
-  for (int it = 0; it < nsweeps; ++it) {
-    const uint32_t n = n0 + (uint32_t)it;
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const uint32_t phase = 2u * n + (uint32_t)c;
-      uint64_t *xme = xb + ((size_t)w * 2 + (phase & 1)) * 4 * G::PL;   // [hs][side][PL]
-      // quad k: update both sites; the new words go to the copy of colour c
-      // (nothing reads it in this phase) and, for the boundary quad, out
-      auto quad = [&](int k, bool boundary) {
-        const int zl = pinned(tid) / G::PL + 2 * k;
-        uint32_t rw[2];
-        if (c == 0) {
-          const uint32_t j = (uint32_t)(col + G::PL * (zw + zl));
-          const u32x4 r = philox_sweep(j, n, d.g, sw, d.key0, d.key1);
-          rw[0] = r.v[0];
-          rw[1] = r.v[1];
-          r23[k][0] = r.v[2];
-          r23[k][1] = r.v[3];
-        } else {
-          rw[0] = r23[k][0];
-          rw[1] = r23[k][1];
-        }
-        uint64_t jc[6], jn[6];
-        record(k, 0, c, jc);
-#pragma unroll
-        for (int hs = 0; hs < 2; ++hs) {
-          if (hs == 0) record(k, 1, c, jn);
-          uint64_t nv[6];
-          neighbours(k, hs, c, nv);
-          update_site<L, FIELD>(sp[k][hs][c], nv, jc, rw[hs], tr);
-#pragma unroll
-          for (int e = 0; e < 6; ++e) jc[e] = jn[e];
-          cb[c * CW + (hs * G::ROWS + zl + 1) * G::PL + col] = sp[k][hs][c];
-          if (P > 1 && boundary)
-            st_coherent(xme, (hs * 2 + (zl == 0 ? 0 : 1)) * G::PL + col, sp[k][hs][c]);
-        }
-      };
-      if (!upper) quad(0, true); else quad(1, true);
-      if (P > 1) {
-        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this thread's planes are out
-        __syncthreads();
-        if (tid == 0)
-          __hip_atomic_store(xf + w, phase + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      if (!upper) quad(1, false); else quad(0, false);
-      if (P > 1) {
-        if (tid == 0) {
-          uint32_t polls = 0;
-          for (;;) {
-            const uint32_t fb = __hip_atomic_load(xf + wb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t fa = __hip_atomic_load(xf + wa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((int32_t)(fb - (phase + 1u)) >= 0 && (int32_t)(fa - (phase + 1u)) >= 0) break;
-            if (++polls > kSplitSpin) {
-              atomicOr(jp.xerr, 1u);
-              break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-          }
-        }
-        __syncthreads();
-        // halo rows of colour c: below <- the top planes of slab pair wb,
-        // above <- the bottom planes of wa; across the wrap the halves trade
-        // places
-        const uint64_t *xbl = xb + ((size_t)wb * 2 + (phase & 1)) * 4 * G::PL;
-        const uint64_t *xab = xb + ((size_t)wa * 2 + (phase & 1)) * 4 * G::PL;
-        for (int i = tid; i < 4 * G::PL; i += G::NT) {
-          const int hs = i / (2 * G::PL), top = (i / G::PL) & 1, cl = i % G::PL;
-          const uint64_t v =
-              top ? ld_coherent(xab, ((hs ^ (w == P - 1 ? 1 : 0)) * 2 + 0) * G::PL + cl)
-                  : ld_coherent(xbl, ((hs ^ (w == 0 ? 1 : 0)) * 2 + 1) * G::PL + cl);
-          cb[c * CW + (hs * G::ROWS + (top ? G::D + 1 : 0)) * G::PL + cl] = v;
-        }
-      } else {
-        __syncthreads();
-        // one work-group holds the whole lattice: the halo rows of one half
-        // are the other half's end planes
-        for (int i = tid; i < 4 * G::PL; i += G::NT) {
-          const int hs = i / (2 * G::PL), top = (i / G::PL) & 1, cl = i % G::PL;
-          cb[c * CW + (hs * G::ROWS + (top ? G::D + 1 : 0)) * G::PL + cl] =
-              cb[c * CW + ((hs ^ 1) * G::ROWS + (top ? 1 : G::D)) * G::PL + cl];
-        }
-      }
-      __syncthreads();
-    }
-  }
-
-  // energies of the final states: U over the colour-0 ends (the copy of
-  // colour 1, halos included, is current), K over both colours
-  uint64_t AU[1][G::PU] = {}, AK[1][G::PK] = {};
-  if (want_energy) {
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-#pragma unroll
-      for (int hs = 0; hs < 2; ++hs) {
-        uint64_t jc[6], nv[6];
-        record(k, hs, 0, jc);
-        neighbours(k, hs, 0, nv);
-        uint64_t u0, u1, u2;
-        bond_count(sp[k][hs][0], nv, jc, u0, u1, u2);
-        vc_add3(AU[0], u0, u1, u2);
-        if (want_energy > 1) {
-          vc_add1(AK[0], sp[k][hs][0]);
-          vc_add1(AK[0], sp[k][hs][1]);
-        }
-      }
-  }
-#pragma unroll
-  for (int k = 0; k < 2; ++k)
-#pragma unroll
-    for (int hs = 0; hs < 2; ++hs)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const int zl = tid / G::PL + 2 * k;
-        st_word(g, c * G::NH + col + G::PL * (zw + zl + hs * G::LH), sp[k][hs][c]);
-      }
-  if (want_energy) {
-    __syncthreads();   // the copies serve as the fold scratch
-    int32_t U[1];
-    wg_count_lanes<G::PU, G::NT, 1>(AU, cb, redi, U);
-    if (tid < 64) {
-      if (P > 1) atomicAdd((int32_t *)(energy + tid), U[0]);
-      else energy[tid] = U[0];
-    }
-    if (want_energy > 1) {
-      __syncthreads();
-      int32_t Kc[1];
-      wg_count_lanes<G::PK, G::NT, 1>(AK, cb, redi, Kc);
-      if (tid < 64) {
-        if (P > 1) atomicAdd((int32_t *)(energy + 64 + tid), Kc[0]);
-        else energy[64 + tid] = Kc[0];
-      }
-    }
-  }
-  // the last work-group out resets the ticket counter for the next launch
-  if (tid == 0 && atomicAdd(ticket + 1, 1u) == gridDim.x - 1) {
-    atomicExch(ticket, 0u);
-    atomicExch(ticket + 1, 0u);
-  }
-}
-
-// ------------------------------------------------- wide split sweep kernel
-// The split sweep for lattices whose slab pair does not fit in LDS with both
-// colours (instantiated for L = 48, P = 4: D = 6 planes per half, 147 KB per
-// colour copy).  Ownership, Philox quads, the exchange protocol, tickets,
-// flags and energies are those of k_split; what differs:
-//  * LDS holds one colour at a time: during the phase of colour c, the copy
-//    of colour 1 - c (own planes plus a halo plane below and above each
-//    half), which is all the phase reads.  The new colour-c words stay in
-//    VGPRs until every wave has finished reading, then replace the copy,
-//    and the neighbours' boundary planes of colour c fill its halo rows.
-//  * thread t owns QPT = D L^2 / 2 NT quads q = t + m NT (m = 0 .. QPT-1),
-//    numbered through the planes in the order 0, D - 1, 1, 2, ..., D - 2,
-//    so that quads m < NB = L^2 / NT lie on the two boundary planes: every
-//    thread updates NB boundary quads, publishes them, then its inner quads.
-//    Each quad's (column, plane) is packed in one VGPR.
-// Reference counterpart: perform_MC_steps, Program/src/eamsim.cu:112-208,
-// whose lattice stays in one block's shared memory (L <= 18 there).
 template <int L, int P, int NT>
-struct WideGeo {
+struct SplitGeo {
   static constexpr int N = L * L * L;
   static constexpr int NH = N / 2;
   static constexpr int LH = L / 2;
@@ -2459,11 +2208,11 @@ struct WideGeo {
 };
 
 template <int L, int P, int NT, bool FIELD>
-__global__ __launch_bounds__(NT) void k_wide(const JobPtrs *__restrict__ jt,
+__global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
                                              const TaskDesc *__restrict__ tasks,
                                              uint32_t *__restrict__ ticket, int nsweeps,
                                              int want_energy) {
-  using G = WideGeo<L, P, NT>;
+  using G = SplitGeo<L, P, NT>;
   __shared__ uint64_t cb[G::CW];            // [half][row][column], one colour
   __shared__ int32_t redi[4 * 64];
   __shared__ uint32_t s_ticket;
@@ -2483,17 +2232,24 @@ __global__ __launch_bounds__(NT) void k_wide(const JobPtrs *__restrict__ jt,
   const int wb = (w + P - 1) % P, wa = (w + 1) % P;   // neighbours below, above
   const int zw = w * G::D;                             // first plane of the lower half
 
-  // quad m: q = tid + m NT; plane index pi = q / PL in the order 0, D-1, 1, ...
-  uint32_t qd[G::QPT];                                 // column | plane << 16
+  // quad m: q = tid + m NT; plane index pi = q / PL in the order 0, D-1, 1,
+  // ...; column col = q mod PL = xh + (L/2) y.  Descriptor: xh | y << 8 |
+  // plane << 16
+  uint32_t qd[G::QPT];
 #pragma unroll
   for (int m = 0; m < G::QPT; ++m) {
-    const int q = tid + m * NT, pi = q / G::PL;
+    const int q = tid + m * NT, pi = q / G::PL, col = q % G::PL;
     const int zl = pi == 0 ? 0 : pi == 1 ? G::D - 1 : pi - 1;
-    qd[m] = (uint32_t)(q % G::PL) | ((uint32_t)zl << 16);
+    qd[m] = (uint32_t)(col % G::LH) | ((uint32_t)(col / G::LH) << 8) | ((uint32_t)zl << 16);
   }
   // read through pinned(): each use recomputes its addresses, so none is
   // held across the sweep loop
-  auto col_of = [&](int m) { return pinned((int)qd[m]) & 0xFFFF; };
+  auto xh_of = [&](int m) { return pinned((int)qd[m]) & 0xFF; };
+  auto y_of = [&](int m) { return (pinned((int)qd[m]) >> 8) & 0xFF; };
+  auto col_of = [&](int m) {
+    const int v = pinned((int)qd[m]);
+    return (v & 0xFF) + G::LH * ((v >> 8) & 0xFF);
+  };
   auto zl_of = [&](int m) { return pinned((int)qd[m]) >> 16; };
   auto row_of = [&](int m, int hs) { return (hs * G::ROWS + zl_of(m) + 1) * G::PL; };
   auto gidx = [&](int m, int hs, int c) {
@@ -2524,20 +2280,22 @@ __global__ __launch_bounds__(NT) void k_wide(const JobPtrs *__restrict__ jt,
   // the six neighbours of own site (m, hs) at colour c, all from the copy of
   // colour 1 - c: +x, -x, +y, -y, +z, -z
   auto neighbours = [&](int m, int hs, int c, uint64_t (&nv)[6]) {
-    const int col = col_of(m), xh = col % G::LH, y = col / G::LH;
-    const int p = (y + zl_of(m) + zw + hs * G::LH + c) & 1;   // x = 2 xh + p
-    const int row = row_of(m, hs);
+    const uint32_t v = (uint32_t)pinned((int)qd[m]);
+    const int xh = (int)(v & 0xFF), y = (int)((v >> 8) & 0xFF), zl = (int)(v >> 16);
+    const int col = xh + G::LH * y;
+    const int p = (y + zl + zw + hs * G::LH + c) & 1;   // x = 2 xh + p
+    const int i = (hs * G::ROWS + zl + 1) * G::PL + col;
     const int xo = p ? (xh == G::LH - 1 ? -(G::LH - 1) : 1) : (xh == 0 ? G::LH - 1 : -1);
-    const uint64_t xl = cb[row + col + xo], xr = cb[row + col];
+    const uint64_t xl = cb[i + xo], xr = cb[i];
     nv[0] = p ? xl : xr;
     nv[1] = p ? xr : xl;
-    nv[2] = cb[row + (y == L - 1 ? col - (L - 1) * G::LH : col + G::LH)];
-    nv[3] = cb[row + (y == 0 ? col + (L - 1) * G::LH : col - G::LH)];
-    nv[4] = cb[row + G::PL + col];
-    nv[5] = cb[row - G::PL + col];
+    nv[2] = cb[i + (y == L - 1 ? -(L - 1) * G::LH : G::LH)];
+    nv[3] = cb[i + (y == 0 ? (L - 1) * G::LH : -G::LH)];
+    nv[4] = cb[i + G::PL];
+    nv[5] = cb[i - G::PL];
   };
   auto record = [&](int m, int hs, int c, uint64_t (&jw)[6]) {
-    site_record(Js, gidx(m, hs, c), true, jw);
+    site_record(Js + (size_t)(c * G::NH + hs * G::LH * G::PL) * 48, col_of(m) + G::PL * (zw + zl_of(m)), true, jw);
   };
 
   const uint32_t sw = (kTagMC << 28) | (d.rt & 0x0FFFFFFFu);
@@ -2572,7 +2330,7 @@ __global__ __launch_bounds__(NT) void k_wide(const JobPtrs *__restrict__ jt,
 #pragma unroll
           for (int e = 0; e < 6; ++e) jc[e] = jn[e];
           if (m < G::NB)
-            st_coherent(xme, (hs * 2 + (zl_of(m) == 0 ? 0 : 1)) * G::PL + col_of(m), sp[m][hs]);
+            st_coherent(xme, hs * 2 * G::PL + pinned(tid) + m * NT, sp[m][hs]);
         }
       };
       // boundary quads, their planes out, the phase published
@@ -2930,8 +2688,8 @@ hipError_t launch_advance(const JobPtrs *jt, int njobs, int nsweeps, hipStream_t
 #ifdef SG_SPLIT_PART
 // The split kernels, in a translation unit of their own (sg_kernels_split.hip).
 int split_slabs(int L, int n_lattices) {
-  // L = 32: four slab pairs of 4 + 4 planes per lattice (k_split), L = 48:
-  // four of 6 + 6 planes (k_wide); one work-group per CU, up to 64 lattices
+  // four slab pairs per lattice (4 + 4 planes each at L = 32, 6 + 6 at
+  // L = 48), one work-group per CU: up to 64 lattices
   return (L == 32 || L == 48) && n_lattices * 4 <= 256 ? 4 : 0;
 }
 
@@ -2940,30 +2698,19 @@ hipError_t launch_split(int L, bool field, const JobPtrs *jt, const TaskDesc *ta
   if (nlat <= 0) return hipSuccess;
   if (nsweeps < 1) return hipErrorInvalidValue;
   const dim3 grid(nlat * P);
-#define SG_SPLIT(l, p)                                                                     \
+#define SG_SPLIT(l, p, nt)                                                                  \
   if (L == l && P == p) {                                                                 \
     if (field)                                                                             \
-      hipLaunchKernelGGL((k_split<l, p, true>), grid, dim3((SplitGeo<l, p>::NT)), 0, st, jt,    \
-                         tasks, ticket, nsweeps, want_energy);                             \
+      hipLaunchKernelGGL((k_split<l, p, nt, true>), grid, dim3(nt), 0, st, jt, tasks, ticket,    \
+                         nsweeps, want_energy);                                            \
     else                                                                                   \
-      hipLaunchKernelGGL((k_split<l, p, false>), grid, dim3((SplitGeo<l, p>::NT)), 0, st, jt,   \
-                         tasks, ticket, nsweeps, want_energy);                             \
+      hipLaunchKernelGGL((k_split<l, p, nt, false>), grid, dim3(nt), 0, st, jt, tasks, ticket,   \
+                         nsweeps, want_energy);                                            \
     return hipGetLastError();                                                              \
   }
-  SG_SPLIT(32, 4)
+  SG_SPLIT(32, 4, 1024)
+  SG_SPLIT(48, 4, 768)
 #undef SG_SPLIT
-#define SG_WIDE(l, p, nt)                                                                  \
-  if (L == l && P == p) {                                                                 \
-    if (field)                                                                             \
-      hipLaunchKernelGGL((k_wide<l, p, nt, true>), grid, dim3(nt), 0, st, jt, tasks, ticket,    \
-                         nsweeps, want_energy);                                            \
-    else                                                                                   \
-      hipLaunchKernelGGL((k_wide<l, p, nt, false>), grid, dim3(nt), 0, st, jt, tasks, ticket,   \
-                         nsweeps, want_energy);                                            \
-    return hipGetLastError();                                                              \
-  }
-  SG_WIDE(48, 4, 768)
-#undef SG_WIDE
   return hipErrorInvalidValue;
 }
 #endif  // SG_SPLIT_PART
