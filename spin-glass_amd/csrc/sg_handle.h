@@ -665,10 +665,10 @@ struct sg_handle {
     dev_alloc(&d_xbuf, n_lat() * split_P * 8 * PL);
     dev_alloc(&d_xflag, n_lat() * split_P);
     dev_alloc(&d_xerr, 1);
-    dev_alloc(&d_ticket, 2);
+    dev_alloc(&d_ticket, kSplitTicketWords);
     HIPCHK(hipMemset(d_xflag, 0, sizeof(uint32_t) * n_lat() * split_P));
     HIPCHK(hipMemset(d_xerr, 0, sizeof(uint32_t)));
-    HIPCHK(hipMemset(d_ticket, 0, sizeof(uint32_t) * 2));
+    HIPCHK(hipMemset(d_ticket, 0, sizeof(uint32_t) * kSplitTicketWords));
   }
 
   // the split launches' phase flags hold absolute phases 2 sweep + c + 1;
@@ -687,7 +687,7 @@ struct sg_handle {
     HIPCHK(hipMemcpy(&e, d_xerr, sizeof(e), hipMemcpyDeviceToHost));
     if (e) {
       HIPCHK(hipMemset(d_xerr, 0, sizeof(uint32_t)));
-      HIPCHK(hipMemset(d_ticket, 0, sizeof(uint32_t) * 2));
+      HIPCHK(hipMemset(d_ticket, 0, sizeof(uint32_t) * kSplitTicketWords));
       clear_split_flags();
       throw Fail{SG_ERR_DEVICE, "split sweeps: a boundary-plane wait timed out (results invalid)"};
     }

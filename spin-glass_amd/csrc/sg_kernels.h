@@ -71,8 +71,10 @@ hipError_t launch_slab(int L, bool field, const JobPtrs *jt, const TaskDesc *tas
 // work-groups resident at once (one per CU), so callers keep n_lattices * P
 // <= 256 and never run two split launches at the same time.
 int split_slabs(int L, int n_lattices);
-// Tasks: lat[0] = lattice descriptor, one per lattice; ticket: two zeroed
-// uint32 counters kept for this task list (the launch leaves them zeroed);
+constexpr int kSplitTicketWords = 8;   // one ticket per slab pair (P <= 7) and a done count
+// Tasks: lat[0] = lattice descriptor, one per lattice; ticket:
+// kSplitTicketWords zeroed uint32 counters kept for this task list (the
+// launch leaves them zeroed);
 // nsweeps >= 1; want_energy as in launch_block.  PT runs before the launch.
 hipError_t launch_split(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int nlat,
                         int P, uint32_t *ticket, int nsweeps, int want_energy, hipStream_t st);

@@ -2146,10 +2146,15 @@ __global__ __launch_bounds__(256) void k_pt_chunk(const JobPtrs *__restrict__ jt
 // parity, and a work-group is never more than one phase ahead of a
 // neighbour.
 //
-// Work-groups take their (lattice, slab pair) from a ticket counter, so
-// the slab pairs of one lattice go to work-groups dispatched together and a
-// lattice never waits on a work-group that cannot start; the last
-// work-group to finish resets the counter for the next launch.  A wait gives
+// Work-group b holds slab pair w = b mod P, so the work-groups of one XCD
+// (b mod 8, the dispatcher's round robin) all hold the same slab pair and
+// its site records fit in that XCD's L2 (1.33 MB per slab pair at L = 48,
+// against 5.3 MB for a whole lattice).  Its lattice comes from the ticket
+// counter of its slab pair, so the k-th work-group of each slab pair to
+// start takes lattice k: the slab pairs of a lattice go to work-groups
+// dispatched together and a lattice never waits on a work-group that cannot
+// start.  The last work-group to finish resets the counters for the next
+// launch.  A wait gives
 // up after kSplitSpin polls (far beyond any normal wait) and sets xerr, which
 // the host turns into an error.
 //
@@ -2201,6 +2206,7 @@ struct SplitGeo {
   static constexpr int PU = ceil_log2(12 * QPT + 1);
   static constexpr int PK = ceil_log2(4 * QPT + 1);
   static_assert(P >= 2 && L % (2 * P) == 0 && D >= 2, "slab pairs with two boundary planes");
+  static_assert(P < kSplitTicketWords, "a ticket per slab pair and a done count");
   static_assert(NT % 64 == 0 && NT <= 1024 && QW % NT == 0 && (2 * PL) % NT == 0,
                 "every thread owns the same number of boundary and inner quads");
   static_assert(CW * 8 + 4 * 64 * 4 + 4 <= 160 * 1024, "the colour copy in LDS");
@@ -2217,10 +2223,10 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
   __shared__ int32_t redi[4 * 64];
   __shared__ uint32_t s_ticket;
   const int tid = threadIdx.x;
-  if (tid == 0) s_ticket = atomicAdd(ticket, 1u);
+  const int w = (int)(blockIdx.x % P);
+  if (tid == 0) s_ticket = atomicAdd(ticket + w, 1u);
   __syncthreads();
-  const uint32_t slot = __builtin_amdgcn_readfirstlane(s_ticket) % gridDim.x;
-  const int ti = (int)(slot / P), w = (int)(slot % P);
+  const int ti = (int)(__builtin_amdgcn_readfirstlane(s_ticket) % (gridDim.x / P));
   const TaskDesc tk = ld_uniform(tasks + ti);
   const JobPtrs jp = ld_uniform(jt + tk.job);
   const LatDesc d = ld_uniform(jp.descs + tk.lat[0]);
@@ -2433,10 +2439,9 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
       if (tid < 64) atomicAdd((int32_t *)(energy + 64 + tid), Kc[0]);
     }
   }
-  // the last work-group out resets the ticket counter for the next launch
-  if (tid == 0 && atomicAdd(ticket + 1, 1u) == gridDim.x - 1) {
-    atomicExch(ticket, 0u);
-    atomicExch(ticket + 1, 0u);
+  // the last work-group out resets the ticket counters for the next launch
+  if (tid == 0 && atomicAdd(ticket + P, 1u) == gridDim.x - 1) {
+    for (int i = 0; i <= P; ++i) atomicExch(ticket + i, 0u);
   }
 }
 
