@@ -61,7 +61,9 @@ typedef struct sg_params {
   int32_t device;        /* HIP device ordinal */
   int32_t form;          /* SG_FORM_* */
   int32_t kernel;        /* SG_KERNEL_*: how the lattices are held during sweeps */
-  int32_t pad;
+  int32_t tile;          /* threads-per-work-group variant of the shape (sg_tune_tiles):
+                            0 = the tuner's pick with SG_KERNEL_AUTO, else the first;
+                            t >= 1 = tile t - 1 (SG_ERR_INVALID if the shape lacks it) */
 } sg_params;
 
 enum {
@@ -76,9 +78,9 @@ enum {
   SG_KERNEL_RESIDENT_SINGLE = 3, /* one lattice per work-group (twice the work-groups of
                                    RESIDENT), PT in a launch of its own; the same as
                                    RESIDENT at L = 22..26, whose kernel holds one lattice */
-  SG_KERNEL_SPLIT = 4     /* L = 32 (<= 256 lattices per launch): a lattice held on
-                             chip by P = 1, 2 or 4 work-groups for a PT interval,
-                             exchanging boundary planes (all P work-groups of a
+  SG_KERNEL_SPLIT = 4     /* L = 32 and 48 (<= 64 lattices per launch): a lattice
+                             held on chip by 4 work-groups for a PT interval,
+                             exchanging boundary planes (all work-groups of a
                              lattice run at once); PT in a launch of its own.  Not
                              used by the batch engine */
 };
@@ -186,6 +188,16 @@ sg_status sg_tune_kernel(int32_t device, int32_t L, int32_t n_lattices, int32_t 
  * where the L has no such shape) */
 sg_status sg_tune_shapes(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
                          int32_t *kernel, double ms[5]);
+/* the same decision down to the tile: each shape is timed at each of its
+ * tiles (threads per work-group: the streamed kernels' 1 or 2 quads or
+ * columns per thread, the split kernel's 1024 or 512 threads at L = 32);
+ * *kernel and *tile (0-based, as sg_params.tile - 1) receive the fastest,
+ * ms[k * 4 + t] the device time of shape k at tile t over 64 sweeps (0
+ * where the L has no such shape or tile).  Cached with sg_tune_shapes. */
+sg_status sg_tune_tiles(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
+                        int32_t *kernel, int32_t *tile, double ms[20]);
+/* the shape and 0-based tile a handle sweeps with */
+sg_status sg_get_shape(const sg_handle *h, int32_t *kernel, int32_t *tile);
 
 /* No counterpart in the reference (it has no entry point of this kind): for
  * callers that cannot allocate device memory — code under HIP graph capture,

@@ -140,6 +140,7 @@ struct Class {
   int kernel = SG_KERNEL_AUTO;
   bool field = false, streamed = false;
   bool single = false;          // one lattice per work-group with a two-lattice kernel
+  int tile = 0;                 // the streamed kernels' tile (the tuner's cached pick)
   hipStream_t st = nullptr;
   std::vector<Job *> jobs;
   bool dirty = true;
@@ -442,15 +443,16 @@ struct sg_batch {
     c.retire_graphs();   // they hold the old tables' pointers and counts
     c.reap_graphs(false);
     if (c.uploaded) HIPCHK(hipEventSynchronize(c.up_ev));  // staging free again
-    int mode = c.kernel;
+    int mode = c.kernel, tile = 0;
     if (mode == SG_KERNEL_AUTO) {
       size_t nlat = 0;
       for (Job *j : c.jobs) nlat += j->h->n_lat();
-      mode = cached_kernel_choice(bp.device, c.L, c.field, nlat);
+      mode = cached_kernel_choice(bp.device, c.L, c.field, nlat, &tile);
     }
     c.streamed = mode == SG_KERNEL_STREAMED;
     c.single = mode == SG_KERNEL_RESIDENT_SINGLE && !single_kernel_supported(c.L);
-    for (Job *j : c.jobs) j->h->set_mode(mode);
+    c.tile = tile;
+    for (Job *j : c.jobs) j->h->set_mode(mode, tile);
     reserve(c, c.jt, c.jobs.size());
     c.jt.n = (int)c.jobs.size();
     for (size_t ji = 0; ji < c.jobs.size(); ++ji) c.jt.h[ji] = c.jobs[ji]->h->self;
@@ -527,14 +529,16 @@ struct sg_batch {
         int mode = kPhUpdate;
         if (it == 0 && ns == pe) mode |= kPhZero;
         if (it == pe - 1) mode |= pm;
-        HIPCHK(launch_slab(c.L, c.field, c.jt.d, c.slabs.d, c.slabs.n, it & 1, it, mode, c.st));
+        HIPCHK(launch_slab(c.L, c.field, c.jt.d, c.slabs.d, c.slabs.n, it & 1, it, mode, c.tile,
+                           c.st));
       }
       for (int it = ns; it < pe; ++it)
         for (int col = 0; col < 2; ++col) {
           int mode = kPhUpdate;
           if (it == ns && col == 0) mode |= kPhZero;
           if (it == pe - 1 && col == 1) mode |= pm;
-          HIPCHK(launch_phase(c.L, c.field, c.jt.d, c.chunks.d, c.chunks.n, col, it, mode, c.st));
+          HIPCHK(launch_phase(c.L, c.field, c.jt.d, c.chunks.d, c.chunks.n, col, it, mode, c.tile,
+                              c.st));
         }
     } else if (single_kernel_supported(c.L)) {
       HIPCHK(launch_pt_chunks(c.jt.d, c.ptc[k].d, c.ptc[k].n, c.st));

@@ -141,14 +141,18 @@ inline uint64_t fnv1a(const void *data, size_t n, uint64_t h = 14695981039346656
 // work-group, twice the work-groups, PT in its own launch) or
 // SG_KERNEL_STREAMED (one launch per colour over chunks of every lattice) —
 // decided by timing each shape the L has on the device; cached per (device,
-// L, field, power-of-two lattice count).  ms[k] receives the measured time of
-// shape k (0 where the L has no such shape).
-int tune_kernel_choice(int device, int L, bool field, size_t n_lattices, double *ms = nullptr);
+// L, field, power-of-two lattice count).  Each shape is timed at each of
+// its tiles (shape_tiles); ms[k] receives the best time of shape k (0 where
+// the L has no such shape), tms[k * kMaxTiles + t] that of tile t, and *tile
+// the winning shape's tile.
+constexpr int kMaxTiles = 4;
+int tune_kernel_choice(int device, int L, bool field, size_t n_lattices, double *ms = nullptr,
+                       int *tile = nullptr, double *tms = nullptr);
 // The same decision without measuring: the cached result when the tuner has
 // measured that (device, L, field, bucket), else the rule its measurements
-// support.  The batch engine decides with this, so no sg_step ever runs the
-// tuner.
-int cached_kernel_choice(int device, int L, bool field, size_t n_lattices);
+// support (tile 0).  The batch engine decides with this, so no sg_step ever
+// runs the tuner.
+int cached_kernel_choice(int device, int L, bool field, size_t n_lattices, int *tile = nullptr);
 
 // Metropolis thresholds of one temperature (see ThrRec): FP64 on the host,
 // floor(2^32 det_exp(-beta dE)) for every (spin bit, u) with dE > 0.
@@ -299,6 +303,7 @@ struct sg_handle {
   // per lattice; boundary-plane exchange buffer, phase flags, error word and
   // the launch's ticket counters
   bool can_split = false, split_shape = false;
+  int tile = 0;                  // the shape's tile (threads per work-group variant)
   int split_P = 0;
   TaskDesc *d_split = nullptr;
   uint64_t *d_xbuf = nullptr;
@@ -612,25 +617,38 @@ struct sg_handle {
     self.thr = d_thr; self.js = d_jobs; self.energy = d_energy; self.counters = d_cnt;
     dev_alloc(&d_self, 1);
     HIPCHK(hipMemcpy(d_self, &self, sizeof(JobPtrs), hipMemcpyHostToDevice));
-    if (p.kernel == SG_KERNEL_AUTO && !arena)
-      set_mode(tune_kernel_choice(device, L, field, n_lat()));
-    else if (p.kernel != SG_KERNEL_AUTO)
-      set_mode(p.kernel);
+    if (p.kernel == SG_KERNEL_AUTO && !arena) {
+      int t = 0;
+      const int k = tune_kernel_choice(device, L, field, n_lat(), nullptr, &t);
+      set_mode(k, t);
+    } else if (p.kernel != SG_KERNEL_AUTO) {
+      set_mode(p.kernel, p.tile > 0 ? p.tile - 1 : 0);
+    }
+  }
+
+  // the tiles (threads per work-group) shape k has at this L: the streamed
+  // kernels kPhaseTiles, the split kernel split_tiles(L), the resident ones 1
+  int shape_tiles(int k) const {
+    if (k == SG_KERNEL_STREAMED) return can_stream ? kPhaseTiles : 0;
+    if (k == SG_KERNEL_SPLIT) return can_split ? split_tiles(L) : 0;
+    return 1;
   }
 
   // switches between the work-group shapes (between PT intervals): k =
   // SG_KERNEL_RESIDENT, _RESIDENT_SINGLE (the same as _RESIDENT where the
-  // resident kernel holds one lattice) or _STREAMED; captured graphs hold the
-  // old launch sequence
-  void set_mode(int k) {
+  // resident kernel holds one lattice), _STREAMED or _SPLIT, at tile t;
+  // captured graphs hold the old launch sequence
+  void set_mode(int k, int t = 0) {
     const bool s = k == SG_KERNEL_STREAMED, one = k == SG_KERNEL_RESIDENT_SINGLE && can_single;
     const bool sp = k == SG_KERNEL_SPLIT;
     if (s && !can_stream) throw Fail{SG_ERR_INVALID, "no streamed kernel for this L"};
     if (sp && !can_split) throw Fail{SG_ERR_INVALID, "no split kernel for this L and size"};
-    if (s == streamed && one == single_shape && sp == split_shape) return;
+    if (t < 0 || t >= shape_tiles(k)) throw Fail{SG_ERR_INVALID, "no such tile for this shape"};
+    if (s == streamed && one == single_shape && sp == split_shape && t == tile) return;
     streamed = s;
     single_shape = one;
     split_shape = sp;
+    tile = t;
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();
   }
@@ -901,7 +919,8 @@ struct sg_handle {
       const int a = per_group ? g * per : 0, n = per_group ? per : (int)n_lat();
       const double fl = (double)seg * N * 64.0 * n;
       timed(0, fl, [&] {
-        HIPCHK(launch_split(L, field, d_self, d_split + a, n, split_P, d_ticket, seg, want_energy, st));
+        HIPCHK(launch_split(L, field, d_self, d_split + a, n, split_P, d_ticket, seg, want_energy,
+                            tile, st));
       });
     }
     HIPCHK(launch_advance(d_self, 1, seg, st));
@@ -912,6 +931,7 @@ struct sg_handle {
   // the final states.  seg = 0 with want_energy: energies only.
   void launch_interval_streamed(int par, int do_pt, int seg, int want_energy) {
     const bool per_group = form == SG_FORM_PLAIN;
+    const int ptile = streamed ? tile : 0;   // the split shape's energy passes: tile 0
     const int emode = want_energy ? (kPhEnergy | (want_energy > 1 ? kPhEnergyK : 0)) : 0;
     for (int g = 0; g < (per_group ? G : 1); ++g) {
       const int ca = per_group ? chunk_off[g] : 0, cb = per_group ? chunk_off[g + 1] : chunk_off[G];
@@ -920,8 +940,8 @@ struct sg_handle {
       if (do_pt) HIPCHK(launch_pt_chunks(d_self, d_ptc[par] + pa, pb - pa, st));
       if (seg == 0) {
         if (want_energy) {
-          HIPCHK(launch_phase(L, field, d_self, d_chunks + ca, cb - ca, 0, 0, kPhZero, st));
-          HIPCHK(launch_phase(L, field, d_self, d_chunks + ca, cb - ca, 1, 0, emode, st));
+          HIPCHK(launch_phase(L, field, d_self, d_chunks + ca, cb - ca, 0, 0, kPhZero, ptile, st));
+          HIPCHK(launch_phase(L, field, d_self, d_chunks + ca, cb - ca, 1, 0, emode, ptile, st));
         }
         continue;
       }
@@ -936,14 +956,14 @@ struct sg_handle {
           int mode = kPhUpdate;
           if (it == 0 && want_energy && ns == seg) mode |= kPhZero;
           if (it == seg - 1) mode |= emode;
-          HIPCHK(launch_slab(L, field, d_self, d_slabs + sa, sb - sa, it & 1, it, mode, st));
+          HIPCHK(launch_slab(L, field, d_self, d_slabs + sa, sb - sa, it & 1, it, mode, ptile, st));
         }
         for (int it = ns; it < seg; ++it)
           for (int c = 0; c < 2; ++c) {
             int mode = kPhUpdate;
             if (it == ns && c == 0 && want_energy) mode |= kPhZero;
             if (it == seg - 1 && c == 1) mode |= emode;
-            HIPCHK(launch_phase(L, field, d_self, d_chunks + ca, cb - ca, c, it, mode, st));
+            HIPCHK(launch_phase(L, field, d_self, d_chunks + ca, cb - ca, c, it, mode, ptile, st));
           }
       });
     }

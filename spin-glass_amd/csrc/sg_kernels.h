@@ -41,7 +41,11 @@ hipError_t launch_init_couplings(int L, uint64_t *J, uint64_t j_off, uint32_t g,
                                  uint32_t key0, uint32_t key1, hipStream_t st);
 
 // ---- streamed lattices: one launch per colour (k_phase), PT in its own launch
-constexpr int kPhaseNT = 256;     // threads per work-group = Philox quads per chunk
+constexpr int kPhaseNT = 256;     // Philox quads per chunk (per task)
+// Tiles of the streamed kernels, picked by the tuner: tile t runs 2^t quads
+// (k_phase) or columns (k_slab) per thread, so kPhaseNT / 2^t threads per
+// k_phase work-group; the tasks are the same for every tile.
+constexpr int kPhaseTiles = 2;
 constexpr int kPtWords = 1024;    // words of each lattice per PT work-group
 enum : int { kPhUpdate = 1, kPhEnergy = 2, kPhEnergyK = 4, kPhZero = 8 };
 
@@ -53,7 +57,7 @@ bool phase_kernel_supported(int L);
 // sweep within the interval (the Philox counter is the job's sweep + it),
 // mode = kPh* bits (energy bits with c = 1 only).
 hipError_t launch_phase(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
-                        int c, int it, int mode, hipStream_t st);
+                        int c, int it, int mode, int tile, hipStream_t st);
 // ---- slab sweeps of streamed lattices: one launch per sweep (k_slab)
 constexpr int kSlabParts = 4;     // slab pairs per lattice (tasks per lattice)
 bool slab_kernel_supported(int L);
@@ -62,7 +66,7 @@ bool slab_kernel_supported(int L);
 // writes the other buffer; it = sweep within the interval; mode = kPh* bits
 // (kPhZero in an earlier launch than the energy bits).
 hipError_t launch_slab(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
-                       int dir, int it, int mode, hipStream_t st);
+                       int dir, int it, int mode, int tile, hipStream_t st);
 
 // ---- split sweeps: a lattice (L = 32, 48) held on chip by P work-groups
 // for a whole PT interval, exchanging boundary planes through jt's xbuf /
@@ -76,8 +80,12 @@ constexpr int kSplitTicketWords = 8;   // one ticket per slab pair (P <= 7) and 
 // kSplitTicketWords zeroed uint32 counters kept for this task list (the
 // launch leaves them zeroed);
 // nsweeps >= 1; want_energy as in launch_block.  PT runs before the launch.
+// split_tiles(L): the tiles (threads per work-group) the split kernel has
+// at L (0: none).
+int split_tiles(int L);
 hipError_t launch_split(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int nlat,
-                        int P, uint32_t *ticket, int nsweeps, int want_energy, hipStream_t st);
+                        int P, uint32_t *ticket, int nsweeps, int want_energy, int tile,
+                        hipStream_t st);
 
 // Tasks: lat[0] = PairDesc index, lat[1] = chunk, job.
 hipError_t launch_pt_chunks(const JobPtrs *jt, const TaskDesc *tasks, int ntasks, hipStream_t st);

@@ -1767,21 +1767,23 @@ __global__ __launch_bounds__(256) void k_build_jsite(uint64_t *__restrict__ Js,
 // U (and K) of the states after this launch (colour 1 only: every bond has
 // one white end, every site index h one white and one black word) into the
 // slots with per-lane atomics.
-template <int L, bool FIELD, bool ENG>
-__global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ jt,
-                                                    const TaskDesc *__restrict__ tasks, int c,
-                                                    int it, int mode) {
+template <int L, bool FIELD, bool ENG, int QPT>
+__global__ __launch_bounds__(kPhaseNT / QPT) void k_phase(const JobPtrs *__restrict__ jt,
+                                                          const TaskDesc *__restrict__ tasks,
+                                                          int c, int it, int mode) {
   using G = Geo<L>;
-  __shared__ uint64_t fold[4 * kPhaseNT];
+  constexpr int NT = kPhaseNT / QPT;        // QPT quads per thread: a tile of the tuner
+  constexpr int PU = ceil_log2(12 * QPT + 1), PK = ceil_log2(4 * QPT + 1);
+  static_assert(kPhaseNT % QPT == 0 && NT % 64 == 0, "whole waves");
+  __shared__ uint64_t fold[PU * NT];
   __shared__ int32_t redi[4 * 64];
   const TaskDesc tk = ld_uniform(tasks + blockIdx.x);
   const JobPtrs jp = ld_uniform(jt + tk.job);
   const LatDesc d = ld_uniform(jp.descs + tk.lat[0]);
   SG_GLOBAL int32_t *__restrict__ energy = to_global(jp.energy) + (size_t)d.e_idx * 128;
   const int tid = threadIdx.x;
-  if ((mode & kPhZero) && tk.lat[1] == 0 && tid < 128) energy[tid] = 0;
-  const int j = (int)tk.lat[1] * kPhaseNT + tid;
-  const bool active = j < G::Q;
+  if ((mode & kPhZero) && tk.lat[1] == 0)
+    for (int i = tid; i < 128; i += NT) energy[i] = 0;
   SG_GLOBAL uint64_t *__restrict__ lat = to_global(jp.spins) + d.spin_off;
   SG_GLOBAL uint64_t *__restrict__ mine = lat + c * G::NH;
   const SG_GLOBAL uint64_t *__restrict__ opp = lat + (1 - c) * G::NH;
@@ -1789,82 +1791,94 @@ __global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ 
       to_global(jp.Js) + 2 * d.j_off + (uint64_t)c * G::NH * 6;
   const bool upd = (mode & kPhUpdate) != 0;
   const bool eng = ENG && (mode & (kPhEnergy | kPhEnergyK)) != 0;
-  // quad j = xh + (L/2)(y + L z) with z < L/2; its sites are (xh, y, z) and
-  // (xh, y, z + L/2): one column, one set of divisions.  Every load is
-  // issued before the Philox rounds, which then hide its latency.
-  const int xh = j % G::LH, y = (j / G::LH) % L, z0 = j / (G::LH * L);
-  const int ry = G::LH * y;
-  const int ryp = G::LH * (y + 1 == L ? 0 : y + 1), rym = G::LH * (y == 0 ? L - 1 : y - 1);
   constexpr int PL = G::LH * L;  // words per plane of one colour
-  const bool work = active && (upd || eng);
-  int hq[2];
-  uint64_t jw[2][6], nv[2][6], sq[2], bq[2] = {0, 0};
-  if (work) {
+  // quad j = xh + (L/2)(y + L z) with z < L/2; its sites are (xh, y, z) and
+  // (xh, y, z + L/2): one column, one set of divisions.  Every load of the
+  // thread's quads is issued before the Philox rounds, which then hide
+  // their latency.
+  int jq[QPT], hq[QPT][2];
+  bool act[QPT];
+  uint64_t jw[QPT][2][6], nv[QPT][2][6], sq[QPT][2], bq[QPT][2];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int z = z0 + q * (L / 2);
-      const int pz = PL * z;
-      const int pzp = PL * (z + 1 == L ? 0 : z + 1), pzm = PL * (z == 0 ? L - 1 : z - 1);
-      const int x = 2 * xh + ((y + z + c) & 1);
-      const int xp = x + 1 == L ? 0 : x + 1, xm = x == 0 ? L - 1 : x - 1;
-      const int h = xh + ry + pz;
-      hq[q] = h;
-      const SG_GLOBAL uint64_t *jr = Js + (size_t)h * 6;
+  for (int k = 0; k < QPT; ++k) {
+    const int j = (int)tk.lat[1] * kPhaseNT + tid + k * NT;
+    jq[k] = j;
+    act[k] = j < G::Q && (upd || eng);
+    const int xh = j % G::LH, y = (j / G::LH) % L, z0 = j / (G::LH * L);
+    const int ry = G::LH * y;
+    const int ryp = G::LH * (y + 1 == L ? 0 : y + 1), rym = G::LH * (y == 0 ? L - 1 : y - 1);
+    bq[k][0] = bq[k][1] = 0;
+    if (act[k]) {
 #pragma unroll
-      for (int k = 0; k < 6; ++k) jw[q][k] = jr[k];
-      nv[q][0] = opp[(xp >> 1) + ry + pz];
-      nv[q][1] = opp[(xm >> 1) + ry + pz];
-      nv[q][2] = opp[xh + ryp + pz];
-      nv[q][3] = opp[xh + rym + pz];
-      nv[q][4] = opp[xh + ry + pzp];
-      nv[q][5] = opp[xh + ry + pzm];
-      sq[q] = mine[h];
-      if (ENG && (mode & kPhEnergyK)) bq[q] = opp[h];
+      for (int q = 0; q < 2; ++q) {
+        const int z = z0 + q * (L / 2);
+        const int pz = PL * z;
+        const int pzp = PL * (z + 1 == L ? 0 : z + 1), pzm = PL * (z == 0 ? L - 1 : z - 1);
+        const int x = 2 * xh + ((y + z + c) & 1);
+        const int xp = x + 1 == L ? 0 : x + 1, xm = x == 0 ? L - 1 : x - 1;
+        const int h = xh + ry + pz;
+        hq[k][q] = h;
+        const SG_GLOBAL uint64_t *jr = Js + (size_t)h * 6;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) jw[k][q][e] = jr[e];
+        nv[k][q][0] = opp[(xp >> 1) + ry + pz];
+        nv[k][q][1] = opp[(xm >> 1) + ry + pz];
+        nv[k][q][2] = opp[xh + ryp + pz];
+        nv[k][q][3] = opp[xh + rym + pz];
+        nv[k][q][4] = opp[xh + ry + pzp];
+        nv[k][q][5] = opp[xh + ry + pzm];
+        sq[k][q] = mine[h];
+        if (ENG && (mode & kPhEnergyK)) bq[k][q] = opp[h];
+      }
     }
   }
-  // the quad's Philox block, drawn by both colour launches of the sweep:
+  // each quad's Philox block, drawn by both colour launches of the sweep:
   // words 2c and 2c + 1 (recomputing it costs less than passing two words
   // per quad through memory from one launch to the next)
-  uint32_t rw[2] = {0, 0};
+  uint32_t rw[QPT][2];
   ThrRec tr{};
-  if (upd) {
-    tr = ld_uniform(jp.thr + d.thr_idx);
-    if (active) {
+  if (upd) tr = ld_uniform(jp.thr + d.thr_idx);
+#pragma unroll
+  for (int k = 0; k < QPT; ++k) {
+    rw[k][0] = rw[k][1] = 0;
+    if (upd && jq[k] < G::Q) {
       const uint32_t n = ld_uniform(jp.js).sweep + (uint32_t)it;
-      const u32x4 r =
-          philox_sweep((uint32_t)j, n, d.g, (kTagMC << 28) | (d.rt & 0x0FFFFFFFu), d.key0, d.key1);
-      rw[0] = c ? r.v[2] : r.v[0];
-      rw[1] = c ? r.v[3] : r.v[1];
+      const u32x4 r = philox_sweep((uint32_t)jq[k], n, d.g, (kTagMC << 28) | (d.rt & 0x0FFFFFFFu),
+                                   d.key0, d.key1);
+      rw[k][0] = c ? r.v[2] : r.v[0];
+      rw[k][1] = c ? r.v[3] : r.v[1];
     }
   }
-  uint64_t AU[1][4] = {{0, 0, 0, 0}}, AK[1][3] = {{0, 0, 0}};
-  if (work) {
+  uint64_t AU[1][PU] = {}, AK[1][PK] = {};
+#pragma unroll
+  for (int k = 0; k < QPT; ++k) {
+    if (!act[k]) continue;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      uint64_t s = sq[q];
+      uint64_t s = sq[k][q];
       if (upd) {
-        update_site<L, FIELD>(s, nv[q], jw[q], rw[q], tr);
-        mine[hq[q]] = s;
+        update_site<L, FIELD>(s, nv[k][q], jw[k][q], rw[k][q], tr);
+        mine[hq[k][q]] = s;
       }
       if (eng) {
         uint64_t u0, u1, u2;
-        bond_count(s, nv[q], jw[q], u0, u1, u2);
+        bond_count(s, nv[k][q], jw[k][q], u0, u1, u2);
         vc_add3(AU[0], u0, u1, u2);
         if (ENG && (mode & kPhEnergyK)) {
           vc_add1(AK[0], s);
-          vc_add1(AK[0], bq[q]);
+          vc_add1(AK[0], bq[k][q]);
         }
       }
     }
   }
   if (eng) {
     int32_t U[1];
-    wg_count_lanes<4, kPhaseNT, 1>(AU, fold, redi, U);
+    wg_count_lanes<PU, NT, 1>(AU, fold, redi, U);
     if (tid < 64) atomicAdd((int32_t *)(energy + tid), U[0]);
     if (ENG && (mode & kPhEnergyK)) {
       __syncthreads();
       int32_t K[1];
-      wg_count_lanes<3, kPhaseNT, 1>(AK, fold, redi, K);
+      wg_count_lanes<PK, NT, 1>(AK, fold, redi, K);
       if (tid < 64) atomicAdd((int32_t *)(energy + 64 + tid), K[0]);
     }
   }
@@ -1888,7 +1902,7 @@ __global__ __launch_bounds__(kPhaseNT) void k_phase(const JobPtrs *__restrict__ 
 // which remain the streamed kernel there.  Energies as in k_phase: zeroed by
 // one launch (kPhZero, task p = 0), accumulated with per-lane atomics by a
 // later one (colour-1 sites: every bond once).
-template <int L>
+template <int L, int CPT_>
 struct SlabGeo {
   static constexpr int N = L * L * L;
   static constexpr int NH = N / 2;
@@ -1897,12 +1911,12 @@ struct SlabGeo {
   static constexpr int NP = kSlabParts;                 // slab pairs per lattice
   static constexpr int S = LH / NP;                     // owned planes per half
   static constexpr int E = S + 2;                       // colour-0 planes per half, with halo
-  static constexpr int CPT = (PL + 1023) / 1024;        // columns per thread
+  static constexpr int CPT = CPT_;                      // columns per thread: a tuner tile
   static constexpr int NT = PL / CPT;                   // every thread has its columns
   static constexpr int C1W = 2 * (S + 4) * PL;          // staged colour-1 words
   static constexpr int PU = ceil_log2(12 * S * CPT + 1);
   static constexpr int PK = ceil_log2(4 * S * CPT + 1);
-  static_assert(LH % NP == 0 && PL % CPT == 0, "slab shape");
+  static_assert(LH % NP == 0 && PL % CPT == 0 && NT % 64 == 0 && NT <= 1024, "slab shape");
   static_assert((2 * E * PL + C1W) * 8 <= 150 * 1024, "both colours' planes in LDS");
 };
 
@@ -1912,11 +1926,11 @@ struct SlabGeo {
 // carries kPhEnergy, and kPhEnergyK for K).  The items of both phases are
 // straight-line code (no per-item branches), so the compiler can issue the
 // loads of later items early.
-template <int L, bool FIELD, bool ENG>
-__global__ __launch_bounds__(SlabGeo<L>::NT) void k_slab(const JobPtrs *__restrict__ jt,
+template <int L, bool FIELD, bool ENG, int CPT>
+__global__ __launch_bounds__((SlabGeo<L, CPT>::NT)) void k_slab(const JobPtrs *__restrict__ jt,
                                                          const TaskDesc *__restrict__ tasks,
                                                          int dir, int it, int mode) {
-  using G = SlabGeo<L>;
+  using G = SlabGeo<L, CPT>;
   static_assert(G::PU * G::NT <= 2 * G::E * G::PL, "fold scratch within the colour-0 planes");
   __shared__ uint64_t c0[2][G::E][G::PL];
   __shared__ uint64_t c1s[G::C1W];
@@ -2587,31 +2601,30 @@ bool phase_kernel_supported(int L) {
 }
 
 hipError_t launch_phase(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
-                        int c, int it, int mode, hipStream_t st) {
+                        int c, int it, int mode, int tile, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
+  if (tile < 0 || tile >= kPhaseTiles) return hipErrorInvalidValue;
   // launches that accumulate energies take the instance with the counting
   // code; the update-only instance holds fewer registers
   const bool eng = (mode & (kPhEnergy | kPhEnergyK)) != 0;
+#define SG_PHASE_LAUNCH(l, f, e, q)                                                        \
+  hipLaunchKernelGGL((k_phase<l, f, e, q>), dim3(ntasks), dim3(kPhaseNT / q), 0, st, jt,   \
+                     tasks, c, it, mode)
+#define SG_PHASE_TILE(l, f, e)                                                             \
+  if (tile == 0) SG_PHASE_LAUNCH(l, f, e, 1); else SG_PHASE_LAUNCH(l, f, e, 2)
   switch (L) {
 #define X(l)                                                                               \
   case l:                                                                                  \
     if (eng) {                                                                             \
-      if (field)                                                                           \
-        hipLaunchKernelGGL((k_phase<l, true, true>), dim3(ntasks), dim3(kPhaseNT), 0, st,   \
-                           jt, tasks, c, it, mode);                                        \
-      else                                                                                 \
-        hipLaunchKernelGGL((k_phase<l, false, true>), dim3(ntasks), dim3(kPhaseNT), 0, st,  \
-                           jt, tasks, c, it, mode);                                        \
-    } else if (field) {                                                                    \
-      hipLaunchKernelGGL((k_phase<l, true, false>), dim3(ntasks), dim3(kPhaseNT), 0, st,    \
-                         jt, tasks, c, it, mode);                                          \
+      if (field) SG_PHASE_TILE(l, true, true); else SG_PHASE_TILE(l, false, true);         \
     } else {                                                                               \
-      hipLaunchKernelGGL((k_phase<l, false, false>), dim3(ntasks), dim3(kPhaseNT), 0, st,   \
-                         jt, tasks, c, it, mode);                                          \
+      if (field) SG_PHASE_TILE(l, true, false); else SG_PHASE_TILE(l, false, false);       \
     }                                                                                      \
     return hipGetLastError();
     SG_PHASE_SIZES(X)
 #undef X
+#undef SG_PHASE_TILE
+#undef SG_PHASE_LAUNCH
     default:
       return hipErrorInvalidValue;
   }
@@ -2629,14 +2642,17 @@ bool slab_kernel_supported(int L) {
 }
 
 hipError_t launch_slab(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int ntasks,
-                       int dir, int it, int mode, hipStream_t st) {
+                       int dir, int it, int mode, int tile, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
   if (!(mode & kPhUpdate)) return hipErrorInvalidValue;   // sweeps only
+  if (tile < 0 || tile >= kPhaseTiles) return hipErrorInvalidValue;
   const bool eng = (mode & (kPhEnergy | kPhEnergyK)) != 0;
   switch (L) {
+#define SG_SLAB_ONE(l, f, e, cpt)                                                          \
+  hipLaunchKernelGGL((k_slab<l, f, e, cpt>), dim3(ntasks), dim3((SlabGeo<l, cpt>::NT)), 0, \
+                     st, jt, tasks, dir, it, mode)
 #define SG_SLAB_LAUNCH(l, f, e)                                                            \
-  hipLaunchKernelGGL((k_slab<l, f, e>), dim3(ntasks), dim3(SlabGeo<l>::NT), 0, st, jt, tasks, \
-                     dir, it, mode)
+  if (tile == 0) SG_SLAB_ONE(l, f, e, 1); else SG_SLAB_ONE(l, f, e, 2)
 #define X(l)                                                                               \
   case l:                                                                                  \
     if (field) {                                                                           \
@@ -2648,6 +2664,7 @@ hipError_t launch_slab(int L, bool field, const JobPtrs *jt, const TaskDesc *tas
     SG_SLAB_SIZES(X)
 #undef X
 #undef SG_SLAB_LAUNCH
+#undef SG_SLAB_ONE
     default:
       return hipErrorInvalidValue;
   }
@@ -2698,13 +2715,17 @@ int split_slabs(int L, int n_lattices) {
   return (L == 32 || L == 48) && n_lattices * 4 <= 256 ? 4 : 0;
 }
 
+int split_tiles(int L) { return L == 32 ? 2 : L == 48 ? 1 : 0; }
+
 hipError_t launch_split(int L, bool field, const JobPtrs *jt, const TaskDesc *tasks, int nlat,
-                        int P, uint32_t *ticket, int nsweeps, int want_energy, hipStream_t st) {
+                        int P, uint32_t *ticket, int nsweeps, int want_energy, int tile,
+                        hipStream_t st) {
   if (nlat <= 0) return hipSuccess;
   if (nsweeps < 1) return hipErrorInvalidValue;
   const dim3 grid(nlat * P);
-#define SG_SPLIT(l, p, nt)                                                                  \
-  if (L == l && P == p) {                                                                 \
+  // tiles: threads per work-group, 1024 then 512 at L = 32; 768 at L = 48
+#define SG_SPLIT(l, p, t, nt)                                                              \
+  if (L == l && P == p && tile == t) {                                                    \
     if (field)                                                                             \
       hipLaunchKernelGGL((k_split<l, p, nt, true>), grid, dim3(nt), 0, st, jt, tasks, ticket,    \
                          nsweeps, want_energy);                                            \
@@ -2713,8 +2734,9 @@ hipError_t launch_split(int L, bool field, const JobPtrs *jt, const TaskDesc *ta
                          nsweeps, want_energy);                                            \
     return hipGetLastError();                                                              \
   }
-  SG_SPLIT(32, 4, 1024)
-  SG_SPLIT(48, 4, 768)
+  SG_SPLIT(32, 4, 0, 1024)
+  SG_SPLIT(32, 4, 1, 512)
+  SG_SPLIT(48, 4, 0, 768)
 #undef SG_SPLIT
   return hipErrorInvalidValue;
 }

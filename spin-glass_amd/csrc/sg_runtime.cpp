@@ -9,17 +9,20 @@ using namespace sg;
 using namespace sg::detail;
 
 // ---------------------------------------------------------------- autotuner
-// Work-group shape of the sweeps for (device, L, field, lattice count rounded
-// up to a power of two): lattice pairs, single lattices (where the resident
-// kernel holds two) and streamed chunks (where the L has a streamed kernel)
-// run the same PT intervals on a scratch handle of that shape (random
-// couplings and spins, 24 temperatures or fewer, 2 replicas, PT every 16,
-// the batched form's captured intervals) and the fastest is kept for the
-// life of the process.
+// Work-group shape and tile of the sweeps for (device, L, field, lattice
+// count rounded up to a power of two): lattice pairs, single lattices (where
+// the resident kernel holds two), streamed chunks (where the L has a
+// streamed kernel; tiles: 1 or 2 quads or columns per thread) and split
+// lattices (where the L has a split kernel and the lattices fit on the chip;
+// tiles: threads per work-group) run the same PT intervals on a scratch
+// handle of that shape (random couplings and spins, 24 temperatures or
+// fewer, 2 replicas, PT every 16, the batched form's captured intervals) and
+// the fastest (shape, tile) is kept for the life of the process.
 namespace {
 struct TuneRes {
-  int kernel;
-  double ms[5];   // by SG_KERNEL_* value; 0 where not measured
+  int kernel, tile;
+  double ms[5];                // by SG_KERNEL_* value: the shape's best tile; 0 where not measured
+  double tms[5][kMaxTiles];    // every (shape, tile)
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, bool, size_t>, TuneRes> g_tune;
@@ -43,15 +46,23 @@ TuneRes tune_measure(int device, int L, bool field, size_t nlat) {
   if (h.can_single) modes.push_back(SG_KERNEL_RESIDENT_SINGLE);
   if (h.can_stream) modes.push_back(SG_KERNEL_STREAMED);
   if (h.can_split) modes.push_back(SG_KERNEL_SPLIT);
-  TuneRes res{SG_KERNEL_RESIDENT, {0, 0, 0, 0, 0}};
-  for (int m : modes) {
-    h.set_mode(m);
-    h.run(64, nullptr, nullptr);  // warm-up: code, caches, the same captured graphs
-    const double before = h.kernel_ms;
-    h.run(64, nullptr, nullptr);
-    res.ms[m] = h.kernel_ms - before;
-    if (res.ms[m] < res.ms[res.kernel]) res.kernel = m;
-  }
+  TuneRes res{SG_KERNEL_RESIDENT, 0, {0, 0, 0, 0, 0}, {}};
+  double best = 0;
+  for (int m : modes)
+    for (int t = 0; t < std::min(h.shape_tiles(m), kMaxTiles); ++t) {
+      h.set_mode(m, t);
+      h.run(64, nullptr, nullptr);  // warm-up: code, caches, the same captured graphs
+      const double before = h.kernel_ms;
+      h.run(64, nullptr, nullptr);
+      const double ms = h.kernel_ms - before;
+      res.tms[m][t] = ms;
+      if (res.ms[m] <= 0 || ms < res.ms[m]) res.ms[m] = ms;
+      if (best <= 0 || ms < best) {
+        best = ms;
+        res.kernel = m;
+        res.tile = t;
+      }
+    }
   return res;
 }
 
@@ -86,23 +97,33 @@ int rule_shape(int L, size_t n_lattices) {
 
 namespace sg {
 namespace detail {
-int tune_kernel_choice(int device, int L, bool field, size_t n_lattices, double *ms) {
+int tune_kernel_choice(int device, int L, bool field, size_t n_lattices, double *ms, int *tile,
+                       double *tms) {
   const TuneRes &r = tune_lookup(device, L, field, n_lattices);
   if (ms) std::copy(r.ms, r.ms + 5, ms);
+  if (tms) std::copy(&r.tms[0][0], &r.tms[0][0] + 5 * kMaxTiles, tms);
+  if (tile) *tile = r.tile;
   return r.kernel;
 }
 
-int cached_kernel_choice(int device, int L, bool field, size_t n_lattices) {
+int cached_kernel_choice(int device, int L, bool field, size_t n_lattices, int *tile) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
+  if (tile) *tile = 0;
   auto it = g_tune.find(std::make_tuple(device, L, field, tune_bucket(n_lattices)));
   if (it == g_tune.end()) return rule_shape(L, n_lattices);
   // the engine's classes run concurrently, so they never take the split
   // shape (its work-groups must run together): the best other shape
   const TuneRes &r = it->second;
-  if (r.kernel != SG_KERNEL_SPLIT) return r.kernel;
-  int best = SG_KERNEL_RESIDENT;
-  for (int k = 1; k < 4; ++k)
-    if (r.ms[k] > 0 && (r.ms[best] <= 0 || r.ms[k] < r.ms[best])) best = k;
+  int best = r.kernel;
+  if (best == SG_KERNEL_SPLIT) {
+    best = SG_KERNEL_RESIDENT;
+    for (int k = 1; k < 4; ++k)
+      if (r.ms[k] > 0 && (r.ms[best] <= 0 || r.ms[k] < r.ms[best])) best = k;
+  }
+  if (tile) {
+    for (int t = 0; t < kMaxTiles; ++t)
+      if (r.tms[best][t] > 0 && r.tms[best][t] <= r.tms[best][*tile]) *tile = t;
+  }
   return best;
 }
 }  // namespace detail
@@ -153,6 +174,7 @@ sg_status sg_create(const sg_params *p, sg_handle **out) {
   if (!p->betas && !(p->T_min > 0.0 && p->T_max > 0.0)) return bad("need betas or T_min/T_max > 0");
   if (p->form != SG_FORM_PLAIN && p->form != SG_FORM_BATCHED) return bad("unknown form");
   if (p->kernel < SG_KERNEL_AUTO || p->kernel > SG_KERNEL_SPLIT) return bad("unknown kernel");
+  if (p->tile < 0) return bad("tile must be >= 0");
   if (p->kernel == SG_KERNEL_STREAMED && !phase_kernel_supported(p->L))
     return bad("no streamed kernel for this L");
   if (p->kernel == SG_KERNEL_SPLIT) {
@@ -321,6 +343,32 @@ sg_status sg_tune_shapes(int32_t device, int32_t L, int32_t n_lattices, int32_t 
     g_create_error = e.msg;
     return e.code;
   }
+}
+
+sg_status sg_tune_tiles(int32_t device, int32_t L, int32_t n_lattices, int32_t field,
+                        int32_t *kernel, int32_t *tile, double ms[20]) {
+  g_create_error.clear();
+  if (!kernel || !tile || !ms || n_lattices < 1 || L < 2 || L > kMaxL || (L & 1))
+    return SG_ERR_INVALID;
+  try {
+    int t = 0;
+    *kernel = tune_kernel_choice(device, L, field != 0, (size_t)n_lattices, nullptr, &t, ms);
+    *tile = t;
+    return SG_OK;
+  } catch (const Fail &e) {
+    g_create_error = e.msg;
+    return e.code;
+  }
+}
+
+sg_status sg_get_shape(const sg_handle *h, int32_t *kernel, int32_t *tile) {
+  if (!h || !kernel || !tile) return SG_ERR_INVALID;
+  *kernel = h->split_shape ? SG_KERNEL_SPLIT
+            : h->streamed  ? SG_KERNEL_STREAMED
+            : h->single_shape ? SG_KERNEL_RESIDENT_SINGLE
+                              : SG_KERNEL_RESIDENT;
+  *tile = h->tile;
+  return SG_OK;
 }
 
 int64_t sg_tune_count(void) {

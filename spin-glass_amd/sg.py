@@ -43,7 +43,7 @@ class _Params(ctypes.Structure):
         ("coupling_seed", ctypes.c_uint64), ("spin_seed", ctypes.c_uint64),
         ("pt_every", ctypes.c_int32), ("meas_every", ctypes.c_int32),
         ("device", ctypes.c_int32), ("form", ctypes.c_int32),
-        ("kernel", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("kernel", ctypes.c_int32), ("tile", ctypes.c_int32),
     ]
 
 
@@ -61,7 +61,7 @@ EXPORTS = [
     "sg_job_footprint", "sg_submit", "sg_step", "sg_drain", "sg_poll", "sg_result",
     "sg_job_times", "sg_batch_stats", "sg_batch_launch_stats", "sg_batch_alloc_stats", "sg_serve",
     "sg_probe_valu", "sg_probe_last_error", "sg_prepare", "sg_graph_count",
-    "sg_tune_count", "sg_run_static",
+    "sg_tune_count", "sg_run_static", "sg_tune_tiles", "sg_get_shape",
 ]
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
@@ -116,6 +116,9 @@ def load_library(path: str = LIB_PATH):
                                  _i32p, _dp, _dp]
     L.sg_tune_shapes.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                  _i32p, _dp]
+    L.sg_tune_tiles.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                _i32p, _i32p, _dp]
+    L.sg_get_shape.argtypes = [_vp, _i32p, _i32p]
     _i64p = ctypes.POINTER(ctypes.c_int64)
     L.sg_batch_params_default.argtypes = [ctypes.POINTER(_BatchParams)]
     L.sg_batch_create.argtypes = [ctypes.POINTER(_BatchParams), ctypes.POINTER(_vp)]
@@ -153,7 +156,8 @@ def _check(status, h=None):
 
 
 def _fill_params(p, L, n_samples, n_replicas, betas, T_range, H, coupling_seed, spin_seed,
-                 pt_every, meas_every, device=0, form=SG_FORM_PLAIN, kernel=SG_KERNEL_AUTO):
+                 pt_every, meas_every, device=0, form=SG_FORM_PLAIN, kernel=SG_KERNEL_AUTO,
+                 tile=0):
     """Fills an sg_params; returns the betas array that p.betas points into
     (keep it alive while p is used)."""
     p.L, p.n_samples, p.n_replicas = L, n_samples, n_replicas
@@ -171,6 +175,7 @@ def _fill_params(p, L, n_samples, n_replicas, betas, T_range, H, coupling_seed, 
     p.coupling_seed, p.spin_seed = coupling_seed, spin_seed
     p.pt_every, p.meas_every, p.device, p.form = pt_every, meas_every, device, form
     p.kernel = kernel
+    p.tile = tile
     return keep
 
 
@@ -179,12 +184,14 @@ class Sim:
 
     def __init__(self, L=16, n_samples=64, n_replicas=2, betas=None, T_range=None, H=0.0,
                  coupling_seed=0x5EED0001, spin_seed=0x5EED0002, pt_every=16,
-                 meas_every=0, device=0, form=SG_FORM_PLAIN, kernel=SG_KERNEL_AUTO):
+                 meas_every=0, device=0, form=SG_FORM_PLAIN, kernel=SG_KERNEL_AUTO, tile=0):
+        """tile: 0 = the tuner's pick (kernel AUTO) or the shape's first tile;
+        t >= 1 = tile t - 1 of the shape (see tune_tiles)."""
         lib = load_library()
         p = _Params()
         self._betas = _fill_params(p, L, n_samples, n_replicas, betas, T_range, H,
                                    coupling_seed, spin_seed, pt_every, meas_every, device, form,
-                                   kernel)
+                                   kernel, tile)
         h = _vp()
         _check(lib.sg_create(ctypes.byref(p), ctypes.byref(h)), None)
         self._h = h
@@ -294,6 +301,13 @@ class Sim:
     def set_profiling(self, on=True):
         self._c(load_library().sg_set_profiling(self._h, 1 if on else 0))
 
+    @property
+    def shape(self):
+        """(SG_KERNEL_* shape, 0-based tile) the handle sweeps with"""
+        k, t = ctypes.c_int32(), ctypes.c_int32()
+        self._c(load_library().sg_get_shape(self._h, ctypes.byref(k), ctypes.byref(t)))
+        return k.value, t.value
+
     def kernel_stats(self, kind=0):
         n = ctypes.c_int64()
         ms = ctypes.c_double()
@@ -338,6 +352,18 @@ def tune_shapes(L, n_lattices, field=False, device=0):
     _check(lib.sg_tune_shapes(device, L, n_lattices, 1 if field else 0, ctypes.byref(k), ms))
     return k.value, {SG_KERNEL_RESIDENT: ms[1], SG_KERNEL_STREAMED: ms[2],
                      SG_KERNEL_RESIDENT_SINGLE: ms[3], SG_KERNEL_SPLIT: ms[4]}
+
+
+def tune_tiles(L, n_lattices, field=False, device=0):
+    """The autotuner's choice down to the tile: (kernel, tile, {(shape,
+    tile): ms}) with every measured (shape, 0-based tile) pair."""
+    lib = load_library()
+    k, t = ctypes.c_int32(), ctypes.c_int32()
+    ms = (ctypes.c_double * 20)()
+    _check(lib.sg_tune_tiles(device, L, n_lattices, 1 if field else 0, ctypes.byref(k),
+                             ctypes.byref(t), ms))
+    return k.value, t.value, {(m, i): ms[m * 4 + i] for m in range(1, 5) for i in range(4)
+                              if ms[m * 4 + i] > 0}
 
 
 STATIC_SCRATCH_BYTES = 32 << 20   # SG_STATIC_SCRATCH_BYTES

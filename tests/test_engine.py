@@ -224,6 +224,17 @@ def test_autotuner_measures_every_shape_and_caches():
     assert k == min((P, O, S, X), key=lambda m: ms[m])
     assert sg.tune_shapes(32, 60) == (k, ms)                     # same power-of-two bucket
     assert sg.tune_kernel(32, 64) == (k, ms[P], ms[S])
+    # tiles: streamed and split have two at L = 32; the choice is the fastest
+    # (shape, tile) and each shape's time is its best tile
+    kt, t, tms = sg.tune_tiles(32, 64)
+    assert kt == k and {(S, 0), (S, 1), (X, 0), (X, 1), (P, 0), (O, 0)} <= set(tms)
+    assert (kt, t) == min(tms, key=tms.get)
+    for m in (P, O, S, X):
+        assert ms[m] == min(v for (mm, _), v in tms.items() if mm == m)
+    sim = sg.Sim(L=32, n_samples=64, n_replicas=2, T_range=(0.3, 2.0, 32),
+                 form=sg.SG_FORM_BATCHED)                        # SG_KERNEL_AUTO: the tuner's pick
+    assert sim.shape == (kt, t)
+    sim.close()
 
 
 @pytest.mark.gpu

@@ -17,10 +17,10 @@ CSEED, SSEED = 0x5EED0001, 0x5EED0002
 
 
 def _pair(L, n_samples, R, betas, H=0.0, pt_every=1, meas_every=0, form=sg.SG_FORM_PLAIN,
-          cseed=CSEED, sseed=SSEED, kernel=sg.SG_KERNEL_AUTO):
+          cseed=CSEED, sseed=SSEED, kernel=sg.SG_KERNEL_AUTO, tile=0):
     sim = sg.Sim(L=L, n_samples=n_samples, n_replicas=R, betas=betas, H=H,
                  coupling_seed=cseed, spin_seed=sseed, pt_every=pt_every,
-                 meas_every=meas_every, form=form, kernel=kernel)
+                 meas_every=meas_every, form=form, kernel=kernel, tile=tile)
     sim.init_lattice()
     orc = oracle.OracleSim(L, sim.G, R, betas, H=H, coupling_seed=cseed, spin_seed=sseed,
                            pt_every=pt_every, meas_every=meas_every)
@@ -127,6 +127,32 @@ def test_split_kernel_matches_oracle(L, H, form):
         np.testing.assert_array_equal(rec, orc.meas(i))
 
 
+@pytest.mark.parametrize("H", [0.5])
+@pytest.mark.parametrize("form", [sg.SG_FORM_PLAIN, sg.SG_FORM_BATCHED])
+def test_split_kernel_second_tile_matches_oracle(form, H):
+    """The split kernel's second tile at L = 32 (512 threads per work-group,
+    four quads each instead of two)."""
+    betas = geometric_betas(0.3, 2.0, 8)
+    sim, orc = _pair(32, 80, 2, betas, H=H, pt_every=4, meas_every=8, form=form,
+                     kernel=sg.SG_KERNEL_SPLIT, tile=2)
+    assert sim.shape == (sg.SG_KERNEL_SPLIT, 1)
+    for n in (4, 9, 11):
+        sim.run(n)
+        orc.run(n)
+        _assert_same(sim, orc)
+    for i, (_, rec) in enumerate(sim.records):
+        np.testing.assert_array_equal(rec, orc.meas(i))
+
+
+def test_missing_tile_is_rejected():
+    betas = geometric_betas(0.3, 2.0, 4)
+    for kw in (dict(L=48, kernel=sg.SG_KERNEL_SPLIT, tile=2),   # one tile at L = 48
+               dict(L=16, kernel=sg.SG_KERNEL_STREAMED, tile=3),
+               dict(L=16, kernel=sg.SG_KERNEL_RESIDENT, tile=2)):
+        with pytest.raises(sg.SGError, match="tile"):
+            sg.Sim(n_samples=64, n_replicas=2, betas=betas, **kw)
+
+
 @pytest.mark.parametrize("kernel", [sg.SG_KERNEL_RESIDENT, sg.SG_KERNEL_STREAMED])
 @pytest.mark.parametrize("H", [0.0, 0.5])
 @pytest.mark.parametrize("L", [30, 34, 36, 38, 42, 44, 46])
@@ -192,14 +218,18 @@ def test_single_lattice_shape_matches_oracle(L, H, form):
         np.testing.assert_array_equal(rec, orc.meas(i))
 
 
+@pytest.mark.parametrize("tile", [1, 2])
 @pytest.mark.parametrize("L", [12, 16, 18, 24, 32, 40, 48])
-def test_streamed_kernel_matches_oracle(L):
+def test_streamed_kernel_matches_oracle(L, tile):
     """One launch per colour over (lattice, chunk) tasks, PT in its own
     launch, energies accumulated by the last colour launch; at L = 32 pairs
     of sweeps run as slab launches (one per sweep, double-buffered spins) and
-    odd leftovers as colour launches: segments of 2 and 1 sweeps here."""
+    odd leftovers as colour launches: segments of 2 and 1 sweeps here.  Both
+    tiles: one or two quads (slab columns) per thread."""
     betas = geometric_betas(0.3, 2.0, 4)
-    sim, orc = _pair(L, 80, 2, betas, pt_every=2, meas_every=3, kernel=sg.SG_KERNEL_STREAMED)
+    sim, orc = _pair(L, 80, 2, betas, pt_every=2, meas_every=3, kernel=sg.SG_KERNEL_STREAMED,
+                     tile=tile)
+    assert sim.shape == (sg.SG_KERNEL_STREAMED, tile - 1)
     sim.run(4)
     orc.run(4)
     _assert_same(sim, orc)
