@@ -253,9 +253,10 @@ def main():
     probe = valu_probe()             # also brings the GPU to its clocks before timing
     b = time_form(SAMPLES_BATCHED, sg.SG_FORM_BATCHED, args.steps, args.warmup)
     value = b["flips"] / b["wall_s"] / 1e9
-    # the work-group shape the autotuner picked for each form (cached per lattice count)
-    shape_b = sg.tune_shapes(L, (SAMPLES_BATCHED // 64) * R * T)
-    shape_p = sg.tune_shapes(L, (SAMPLES_PLAIN // 64) * R * T)
+    # the work-group shape and tile the autotuner picked for each form (cached per lattice
+    # count), with the device ms of every (shape, tile) it timed
+    shape_b = sg.tune_tiles(L, (SAMPLES_BATCHED // 64) * R * T)
+    shape_p = sg.tune_tiles(L, (SAMPLES_PLAIN // 64) * R * T)
     k = time_form(SAMPLES_BATCHED, sg.SG_FORM_BATCHED, args.steps, args.warmup, profile=True)
     plain = plain_same = None
     if not args.no_plain:
@@ -304,11 +305,12 @@ def main():
             "plain_workload": f"same, {SAMPLES_PLAIN} samples (one group: the drop-in shape, "
                               f"where batched and plain are the same launch)",
             "batched_over_plain": None if plain is None else round(value / plain, 3),
-            "shape": {"batched": SHAPE_NAMES[shape_b[0]], "plain": SHAPE_NAMES[shape_p[0]],
-                      "tuner_ms_batched": {SHAPE_NAMES[m]: round(v, 4)
-                                           for m, v in shape_b[1].items() if v > 0},
-                      "tuner_ms_plain": {SHAPE_NAMES[m]: round(v, 4)
-                                         for m, v in shape_p[1].items() if v > 0}},
+            "shape": {"batched": f"{SHAPE_NAMES[shape_b[0]]}, tile {shape_b[1]}",
+                      "plain": f"{SHAPE_NAMES[shape_p[0]]}, tile {shape_p[1]}",
+                      "tuner_ms_batched": {f"{SHAPE_NAMES[m]}/{t}": round(v, 4)
+                                           for (m, t), v in shape_b[2].items()},
+                      "tuner_ms_plain": {f"{SHAPE_NAMES[m]}/{t}": round(v, 4)
+                                         for (m, t), v in shape_p[2].items()}},
             "serve": serve,
             "parallelism": "1 GPU",
         },

@@ -17,15 +17,17 @@ PT_EVERY = 16
 VALU_PEAK_TOPS = 39.3
 HBM_PEAK_GBS = 8000.0
 W_ALG = 1.22
-# HBM-side bytes per launch of the large-lattice kernels at 64 lattices (FETCH_SIZE +
-# WRITE_SIZE, scaled by a copy of known size), keyed by (L, shape): k_split<32,4> per 16-sweep
-# launch (mostly the boundary-plane exchange), k_slab<32> per sweep launch, k_phase<48> per
-# colour launch; profiles/k_split_L32_k_phase_L48_64lattices.txt
-TRAFFIC_LARGE = {(32, sg.SG_KERNEL_SPLIT): 317.0e6, (32, sg.SG_KERNEL_STREAMED): 37.96e6,
-                 (48, sg.SG_KERNEL_STREAMED): 109.47e6}
-LARGE_KERNELS = {(32, sg.SG_KERNEL_SPLIT): "k_split<32,4,false> (16 sweeps per launch)",
-                 (32, sg.SG_KERNEL_STREAMED): "k_slab<32,false> (one sweep per launch)",
-                 (48, sg.SG_KERNEL_STREAMED): "k_phase<48,false> (one colour per launch)"}
+# Memory-side bytes per launch of the large-lattice kernels at 64 lattices (FETCH_SIZE +
+# WRITE_SIZE, scaled by a copy of known size), keyed by (L, shape, tile): the split kernel per
+# 16-sweep launch (most of it the boundary-plane exchange); profiles/k_split_L32_L48_64lattices.txt
+TRAFFIC_LARGE = {(32, sg.SG_KERNEL_SPLIT, 0): 310.4e6, (48, sg.SG_KERNEL_SPLIT, 0): 768.3e6}
+LARGE_KERNELS = {(32, sg.SG_KERNEL_SPLIT, 0): "k_split<32,4,1024,false> (16 sweeps per launch)",
+                 (32, sg.SG_KERNEL_SPLIT, 1): "k_split<32,4,512,false> (16 sweeps per launch)",
+                 (48, sg.SG_KERNEL_SPLIT, 0): "k_split<48,4,768,false> (16 sweeps per launch)",
+                 (32, sg.SG_KERNEL_STREAMED, 0): "k_slab<32,false,false,1> (one sweep per launch)",
+                 (32, sg.SG_KERNEL_STREAMED, 1): "k_slab<32,false,false,2> (one sweep per launch)",
+                 (48, sg.SG_KERNEL_STREAMED, 0): "k_phase<48,false,false,1> (one colour per launch)",
+                 (48, sg.SG_KERNEL_STREAMED, 1): "k_phase<48,false,false,2> (one colour per launch)"}
 L_CHOICES = (8, 12, 16, 20, 24)
 
 
@@ -105,9 +107,9 @@ def large(args):
     shapes = {}
     tot_flips = tot_s = 0.0
     for L in (32, 48):
-        shapes[L] = sg.tune_kernel(L, 64)[0]   # the shape SG_KERNEL_AUTO runs (cached)
         s = sg.Sim(L=L, n_samples=64, n_replicas=2, T_range=(0.3, 2.0, 32), pt_every=PT_EVERY,
                    meas_every=1024, form=sg.SG_FORM_BATCHED)
+        shapes[L] = s.shape          # the (shape, tile) SG_KERNEL_AUTO picked (the tuner's)
         s.keep_records = False
         s.init_lattice()
         s.run(2 * PT_EVERY)
@@ -131,10 +133,10 @@ def large(args):
     cfg = {"workload": f"L=32 and L=48 (separately), 64 samples, 32 temperatures geometric in "
                        f"T=[0.3,2.0], 2 replicas, PT every {PT_EVERY}, {sweeps} sweeps each",
            "per_L": per_l, "form": "batched", "parallelism": "1 GPU"}
-    r = _valu_roofline(value, ", ".join(LARGE_KERNELS.get((L, shapes[L]), f"L={L} shape "
+    r = _valu_roofline(value, ", ".join(LARGE_KERNELS.get((L, *shapes[L]), f"L={L} shape "
                                                            f"{shapes[L]}") for L in per_l))
     r["hbm"] = {L: v["hbm_frac"] for L, v in per_l.items()}
-    r["traffic"] = {L: TRAFFIC_LARGE.get((L, shapes[L])) for L in per_l}
+    r["traffic"] = {L: TRAFFIC_LARGE.get((L, *shapes[L])) for L in per_l}
     return _base("spin flips/ns, L=32 and 48", value, "spin flips/ns", tot_s, cfg, r,
                  steps=2 * sweeps // PT_EVERY)
 
