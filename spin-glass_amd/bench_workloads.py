@@ -195,6 +195,19 @@ def serve_run(n_jobs, rate, seed=0x5EED0005):
     lat = np.array(lat)
     flips = sum(j.flips() for j in jobs)
     offered = flips / arrivals[-1] / 1e9 if arrivals[-1] > 0 else float("nan")
+    # the floor under the tail: the stream's longest job (most sweeps x L^3) run
+    # alone on an idle engine, submission to result
+    big = max(range(n_jobs), key=lambda i: jobs[i].n_sweeps * jobs[i].L ** 3)
+    j = jobs[big]
+    b = sg.Batch(pt_every=PT_EVERY, slice_pt_steps=2, form=sg.SG_FORM_BATCHED,
+                 arena_bytes=16 << 30)
+    i = b.submit(sg.Job(L=j.L, n_sweeps=j.n_sweeps, n_samples=64, T_range=j.T_range,
+                        coupling_seed=j.coupling_seed, spin_seed=j.spin_seed,
+                        meas_every=1024))
+    b.drain()
+    t_sub, _, t_done = b.times(int(i))
+    b.result(i)
+    b.close()
     return {"jobs": n_jobs, "rate_per_s": rate,
             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
@@ -202,7 +215,9 @@ def serve_run(n_jobs, rate, seed=0x5EED0005):
             "served_flips_per_ns": round(flips / wall / 1e9, 1),
             "offered_flips_per_ns": round(offered, 1),
             "arrival_window_s": round(float(arrivals[-1]), 3),
-            "wall_s": round(wall, 3)}
+            "wall_s": round(wall, 3),
+            "longest_job_alone_s": round(t_done - t_sub, 4),
+            "longest_job": f"L={j.L}, {j.n_sweeps} sweeps, {j.T_range[2]} T, 64 samples"}
 
 
 def serve(args):
