@@ -63,7 +63,9 @@ typedef struct sg_params {
   int32_t kernel;        /* SG_KERNEL_*: how the lattices are held during sweeps */
   int32_t tile;          /* threads-per-work-group variant of the shape (sg_tune_tiles):
                             0 = the tuner's pick with SG_KERNEL_AUTO, else the first;
-                            t >= 1 = tile t - 1 (SG_ERR_INVALID if the shape lacks it) */
+                            t >= 1 = tile t - 1 (SG_ERR_INVALID if the shape lacks it).
+                            Batch-engine jobs run their class's tile: the tuner's
+                            cached one for SG_KERNEL_AUTO, else the first */
 } sg_params;
 
 enum {
