@@ -373,10 +373,12 @@ def test_benchmark_workload_at_full_size_matches_oracle():
 def test_large_lattice_benchmark_at_full_size_matches_oracle(L):
     """The large-lattice benchmark's shape at its full size: 64 samples, 32
     temperatures geometric in T = [0.3, 2], 2 replicas (64 lattices), PT every
-    16, batched form with the tuner's shape (slab sweeps at L = 32, colour
-    launches at L = 48).  Two PT intervals and a measurement, bit-exact."""
+    16, batched form with the tuner's shape and tile (the split kernel at
+    both sizes on the MI355X).  Two PT intervals and a measurement, bit-exact."""
     betas = geometric_betas(0.3, 2.0, 32)
     sim, orc = _pair(L, 64, 2, betas, pt_every=16, meas_every=32, form=sg.SG_FORM_BATCHED)
+    k, t, _ = sg.tune_tiles(L, 64)
+    assert sim.shape == (k, t)
     sim.run(32)
     orc.run(32)
     _assert_same(sim, orc)
