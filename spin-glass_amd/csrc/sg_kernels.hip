@@ -2215,6 +2215,8 @@ struct SplitGeo {
   static constexpr int QW = D * PL;             // quads per slab pair
   static constexpr int QPT = QW / NT;           // quads per thread
   static constexpr int NB = 2 * PL / NT;        // of them on the boundary planes: m < NB
+  static constexpr int HPT = 4 * PL / NT;       // halo words per thread and phase
+  static_assert(PL % 64 == 0, "a wave's halo words in one row");
   static constexpr int ROWS = D + 2;            // halo, D own planes, halo
   static constexpr int CW = 2 * ROWS * PL;      // the colour copy [half][row][column]
   static constexpr int PU = ceil_log2(12 * QPT + 1);
@@ -2227,7 +2229,7 @@ struct SplitGeo {
   static_assert(PU * NT <= CW, "fold scratch in the copy");
 };
 
-template <int L, int P, int NT, bool FIELD>
+template <int L, int P, int NT, bool HALO_FIRST, bool FIELD>
 __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
                                              const TaskDesc *__restrict__ tasks,
                                              uint32_t *__restrict__ ticket, int nsweeps,
@@ -2253,27 +2255,30 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
   const int zw = w * G::D;                             // first plane of the lower half
 
   // quad m: q = tid + m NT; plane index pi = q / PL in the order 0, D-1, 1,
-  // ...; column col = q mod PL = xh + (L/2) y.  Descriptor: xh | y << 8 |
-  // plane << 16
+  // ...; column col = q mod PL = xh + (L/2) y.  Descriptor: bits 0-16 the
+  // byte offset 8 (zl PL + col) of its word among a half's own rows; bit 17
+  // (y + zl) & 1; bits 18, 19: xh = 0, xh = L/2 - 1; bits 20, 21: y = 0,
+  // y = L - 1
+  static_assert(8 * G::QW < (1 << 17), "descriptor byte offsets");
   uint32_t qd[G::QPT];
 #pragma unroll
   for (int m = 0; m < G::QPT; ++m) {
     const int q = tid + m * NT, pi = q / G::PL, col = q % G::PL;
     const int zl = pi == 0 ? 0 : pi == 1 ? G::D - 1 : pi - 1;
-    qd[m] = (uint32_t)(col % G::LH) | ((uint32_t)(col / G::LH) << 8) | ((uint32_t)zl << 16);
+    const int xh = col % G::LH, y = col / G::LH;
+    qd[m] = (uint32_t)(8 * (zl * G::PL + col)) | ((uint32_t)((y + zl) & 1) << 17) |
+            ((uint32_t)(xh == 0) << 18) | ((uint32_t)(xh == G::LH - 1) << 19) |
+            ((uint32_t)(y == 0) << 20) | ((uint32_t)(y == L - 1) << 21);
   }
   // read through pinned(): each use recomputes its addresses, so none is
   // held across the sweep loop
-  auto xh_of = [&](int m) { return pinned((int)qd[m]) & 0xFF; };
-  auto y_of = [&](int m) { return (pinned((int)qd[m]) >> 8) & 0xFF; };
-  auto col_of = [&](int m) {
-    const int v = pinned((int)qd[m]);
-    return (v & 0xFF) + G::LH * ((v >> 8) & 0xFF);
-  };
-  auto zl_of = [&](int m) { return pinned((int)qd[m]) >> 16; };
-  auto row_of = [&](int m, int hs) { return (hs * G::ROWS + zl_of(m) + 1) * G::PL; };
+  auto off_of = [&](int m) { return pinned((int)qd[m]) & 0x1FFF8; };
+  // the copy's word at a byte offset; own site (m, hs) sits in row zl + 1 of
+  // half hs
+  auto cbw = [&](int off) -> uint64_t & { return *(uint64_t *)((char *)cb + off); };
+  auto cb_off = [&](int m, int hs) { return off_of(m) + (hs * G::ROWS + 1) * G::PL * 8; };
   auto gidx = [&](int m, int hs, int c) {
-    return c * G::NH + col_of(m) + G::PL * (zw + zl_of(m) + hs * G::LH);
+    return c * G::NH + (off_of(m) >> 3) + G::PL * (zw + hs * G::LH);
   };
 
   // the registers hold the colour the phase updates (colour 0 first), the
@@ -2284,7 +2289,7 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
 #pragma unroll
     for (int hs = 0; hs < 2; ++hs) {
       sp[m][hs] = ld_word(g, gidx(m, hs, 0));
-      cb[row_of(m, hs) + col_of(m)] = ld_word(g, gidx(m, hs, 1));
+      cbw(cb_off(m, hs)) = ld_word(g, gidx(m, hs, 1));
     }
   for (int i = tid; i < 4 * G::PL; i += NT) {
     const int hs = i / (2 * G::PL), top = (i / G::PL) & 1, cl = i % G::PL;
@@ -2297,25 +2302,43 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
     atomicExch((int32_t *)(energy + tid), 0);
   __syncthreads();
 
-  // the six neighbours of own site (m, hs) at colour c, all from the copy of
-  // colour 1 - c: +x, -x, +y, -y, +z, -z
-  auto neighbours = [&](int m, int hs, int c, uint64_t (&nv)[6]) {
-    const uint32_t v = (uint32_t)pinned((int)qd[m]);
-    const int xh = (int)(v & 0xFF), y = (int)((v >> 8) & 0xFF), zl = (int)(v >> 16);
-    const int col = xh + G::LH * y;
-    const int p = (y + zl + zw + hs * G::LH + c) & 1;   // x = 2 xh + p
-    const int i = (hs * G::ROWS + zl + 1) * G::PL + col;
-    const int xo = p ? (xh == G::LH - 1 ? -(G::LH - 1) : 1) : (xh == 0 ? G::LH - 1 : -1);
-    const uint64_t xl = cb[i + xo], xr = cb[i];
-    nv[0] = p ? xl : xr;
-    nv[1] = p ? xr : xl;
-    nv[2] = cb[i + (y == L - 1 ? -(L - 1) * G::LH : G::LH)];
-    nv[3] = cb[i + (y == 0 ? (L - 1) * G::LH : -G::LH)];
-    nv[4] = cb[i + G::PL];
-    nv[5] = cb[i - G::PL];
+  // byte offsets in the copy (of colour 1 - c) of the neighbours of own site
+  // (m, 0) at colour c: +x, -x, +y, -y; +z and -z are the words PL rows
+  // above and below a.  Half 1 adds ROWS PL words to each.  x = 2 xh + p,
+  // p = (y + z + c) & 1 with z = zw + zl + hs L/2 (L/2 even: p is the same in
+  // both halves); one x-neighbour shares the word index, the other is one
+  // word away or across the row's wrap.
+  struct NbOff {
+    int a, px, mx, py, my;
   };
+  auto nb_off = [&](int m, int c) {
+    const uint32_t v = (uint32_t)pinned((int)qd[m]);
+    const int a = (int)(v & 0x1FFF8u) + G::PL * 8;
+    const bool p = (((v >> 17) ^ (uint32_t)(zw + c)) & 1u) != 0;
+    const int t = (int)((v >> (p ? 19 : 18)) & 1u);   // the far x-neighbour wraps
+    const int u = t * (G::LH * 8) - 8;                 // p = 0: it is at a + u, p = 1: a - u
+    NbOff r;
+    r.a = a;
+    r.px = p ? a - u : a;
+    r.mx = p ? a : a + u;
+    r.py = a + G::LH * 8 - (int)((v >> 21) & 1u) * (L * G::LH * 8);
+    r.my = a - G::LH * 8 + (int)((v >> 20) & 1u) * (L * G::LH * 8);
+    return r;
+  };
+  auto neighbours = [&](const NbOff &r, int hs, uint64_t (&nv)[6]) {
+    const int h = hs * G::ROWS * G::PL * 8;
+    nv[0] = cbw(r.px + h);
+    nv[1] = cbw(r.mx + h);
+    nv[2] = cbw(r.py + h);
+    nv[3] = cbw(r.my + h);
+    nv[4] = cbw(r.a + h + G::PL * 8);
+    nv[5] = cbw(r.a + h - G::PL * 8);
+  };
+  // own site (m, hs)'s coupling record at colour c: record index
+  // zl PL + col of the slab's half, 48 bytes each
   auto record = [&](int m, int hs, int c, uint64_t (&jw)[6]) {
-    site_record(Js + (size_t)(c * G::NH + hs * G::LH * G::PL) * 48, col_of(m) + G::PL * (zw + zl_of(m)), true, jw);
+    site_record(Js + (size_t)(c * G::NH + hs * G::LH * G::PL + G::PL * zw) * 48,
+                off_of(m) >> 3, true, jw);
   };
 
   const uint32_t sw = (kTagMC << 28) | (d.rt & 0x0FFFFFFFu);
@@ -2329,7 +2352,7 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
       auto quad = [&](int m) {
         uint32_t rw[2];
         if (c == 0) {
-          const uint32_t j = (uint32_t)(col_of(m) + G::PL * (zw + zl_of(m)));
+          const uint32_t j = (uint32_t)((off_of(m) >> 3) + G::PL * zw);
           const u32x4 r = philox_sweep(j, n, d.g, sw, d.key0, d.key1);
           rw[0] = r.v[0];
           rw[1] = r.v[1];
@@ -2341,11 +2364,12 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
         }
         uint64_t jc[6], jn[6];
         record(m, 0, c, jc);
+        const NbOff nb = nb_off(m, c);
 #pragma unroll
         for (int hs = 0; hs < 2; ++hs) {
           if (hs == 0) record(m, 1, c, jn);
           uint64_t nv[6];
-          neighbours(m, hs, c, nv);
+          neighbours(nb, hs, nv);
           update_site<L, FIELD>(sp[m][hs], nv, jc, rw[hs], tr);
 #pragma unroll
           for (int e = 0; e < 6; ++e) jc[e] = jn[e];
@@ -2364,45 +2388,74 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
         __hip_atomic_store(xf + w, phase + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       // inner quads while the planes travel
 #pragma unroll
-      for (int m = G::NB; m < G::QPT; ++m) {
-        quad(m);
-      }
-      __syncthreads();                      // every read of the colour 1 - c copy is done
-      // the new colour-c words into the copy, the colour 1 - c words (the
-      // next phase's) out of it
-#pragma unroll
-      for (int m = 0; m < G::QPT; ++m)
-#pragma unroll
-        for (int hs = 0; hs < 2; ++hs) {
-          const int pos = row_of(m, hs) + col_of(m);
-          const uint64_t o = cb[pos];
-          cb[pos] = sp[m][hs];
-          sp[m][hs] = o;
-        }
-      if (tid == 0) {
+      for (int m = G::NB; m < G::QPT; ++m) quad(m);
+      __syncthreads();   // every read of the colour 1 - c copy is done
+      // the neighbours' planes of this phase: halo rows of colour c, below <-
+      // the top planes of slab pair wb, above <- the bottom planes of wa;
+      // across the wrap the halves trade places.  poll: until both flags
+      // show this phase
+      const uint64_t *xbl = xb + ((size_t)wb * 2 + (phase & 1)) * 4 * G::PL;
+      const uint64_t *xab = xb + ((size_t)wa * 2 + (phase & 1)) * 4 * G::PL;
+      auto poll = [&](auto uniform) {
         uint32_t polls = 0;
         for (;;) {
-          const uint32_t fb = __hip_atomic_load(xf + wb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          const uint32_t fa = __hip_atomic_load(xf + wa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const uint32_t fb = uniform(__hip_atomic_load(xf + wb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+          const uint32_t fa = uniform(__hip_atomic_load(xf + wa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
           if ((int32_t)(fb - (phase + 1u)) >= 0 && (int32_t)(fa - (phase + 1u)) >= 0) break;
           if (++polls > kSplitSpin) {
-            atomicOr(jp.xerr, 1u);
+            if ((tid & 63) == 0) atomicOr(jp.xerr, 1u);
             break;
           }
           __builtin_amdgcn_s_sleep(1);
         }
-      }
-      __syncthreads();
-      // halo rows of colour c: below <- the top planes of slab pair wb, above
-      // <- the bottom planes of wa; across the wrap the halves trade places
-      const uint64_t *xbl = xb + ((size_t)wb * 2 + (phase & 1)) * 4 * G::PL;
-      const uint64_t *xab = xb + ((size_t)wa * 2 + (phase & 1)) * 4 * G::PL;
-      for (int i = tid; i < 4 * G::PL; i += NT) {
-        const int hs = i / (2 * G::PL), top = (i / G::PL) & 1, cl = i % G::PL;
-        const uint64_t v =
-            top ? ld_coherent(xab, ((hs ^ (w == P - 1 ? 1 : 0)) * 2 + 0) * G::PL + cl)
-                : ld_coherent(xbl, ((hs ^ (w == 0 ? 1 : 0)) * 2 + 1) * G::PL + cl);
-        cb[(hs * G::ROWS + (top ? G::D + 1 : 0)) * G::PL + cl] = v;
+      };
+      // halo word k of this thread (fetched before the swap): i = tid + k NT;
+      // the 64 words of a wave lie in one row (PL is a multiple of 64), so
+      // the row arithmetic is the wave's (scalar) and a lane adds its index
+      const int wave_i = __builtin_amdgcn_readfirstlane(tid) & ~63;
+      auto halo_word = [&](int k) {
+        const int i0 = wave_i + k * NT, hs = i0 / (2 * G::PL), top = (i0 / G::PL) & 1;
+        const uint64_t *src = top ? xab + ((hs ^ (w == P - 1 ? 1 : 0)) * 2 + 0) * G::PL
+                                  : xbl + ((hs ^ (w == 0 ? 1 : 0)) * 2 + 1) * G::PL;
+        return ld_coherent(src + i0 % G::PL, tid & 63);
+      };
+      auto halo_slot = [&](int k) -> uint64_t & {
+        const int i0 = wave_i + k * NT, hs = i0 / (2 * G::PL), top = (i0 / G::PL) & 1;
+        return cb[(hs * G::ROWS + (top ? G::D + 1 : 0)) * G::PL + i0 % G::PL + (tid & 63)];
+      };
+      // the new colour-c words into the copy, the colour 1 - c words (the
+      // next phase's) out of it
+      auto swap = [&] {
+#pragma unroll
+        for (int m = 0; m < G::QPT; ++m)
+#pragma unroll
+          for (int hs = 0; hs < 2; ++hs) {
+            uint64_t &slot = cbw(cb_off(m, hs));
+            const uint64_t o = slot;
+            slot = sp[m][hs];
+            sp[m][hs] = o;
+          }
+      };
+      if constexpr (HALO_FIRST) {
+        // every wave polls, then its halo words are in flight during the swap
+        poll([](uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); });
+        uint64_t hv[G::HPT];
+#pragma unroll
+        for (int k = 0; k < G::HPT; ++k) hv[k] = halo_word(k);
+        swap();
+#pragma unroll
+        for (int k = 0; k < G::HPT; ++k) halo_slot(k) = hv[k];
+      } else {
+        // thread 0 polls while the others swap
+        swap();
+        if (tid == 0) poll([](uint32_t v) { return v; });
+        __syncthreads();
+        for (int i = tid; i < 4 * G::PL; i += NT) {
+          const int hs = i / (2 * G::PL), top = (i / G::PL) & 1, cl = i % G::PL;
+          cb[(hs * G::ROWS + (top ? G::D + 1 : 0)) * G::PL + cl] =
+              top ? ld_coherent(xab, ((hs ^ (w == P - 1 ? 1 : 0)) * 2 + 0) * G::PL + cl)
+                  : ld_coherent(xbl, ((hs ^ (w == 0 ? 1 : 0)) * 2 + 1) * G::PL + cl);
+        }
       }
       __syncthreads();
     }
@@ -2418,7 +2471,7 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
       for (int hs = 0; hs < 2; ++hs) {
         uint64_t jc[6], nv[6];
         record(m, hs, 0, jc);
-        neighbours(m, hs, 0, nv);
+        neighbours(nb_off(m, 0), hs, nv);
         uint64_t u0, u1, u2;
         bond_count(sp[m][hs], nv, jc, u0, u1, u2);
         vc_add3(AU[0], u0, u1, u2);
@@ -2431,7 +2484,7 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
 #pragma unroll
       for (int hs = 0; hs < 2; ++hs) {
         vc_add1(AK[0], sp[m][hs]);
-        vc_add1(AK[0], cb[row_of(m, hs) + col_of(m)]);
+        vc_add1(AK[0], cbw(cb_off(m, hs)));
       }
   }
 #pragma unroll
@@ -2439,7 +2492,7 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
 #pragma unroll
     for (int hs = 0; hs < 2; ++hs) {
       st_word(g, gidx(m, hs, 0), sp[m][hs]);
-      st_word(g, gidx(m, hs, 1), cb[row_of(m, hs) + col_of(m)]);
+      st_word(g, gidx(m, hs, 1), cbw(cb_off(m, hs)));
     }
   if (want_energy) {
     __syncthreads();   // the copy serves as the fold scratch
@@ -2723,20 +2776,23 @@ hipError_t launch_split(int L, bool field, const JobPtrs *jt, const TaskDesc *ta
   if (nlat <= 0) return hipSuccess;
   if (nsweeps < 1) return hipErrorInvalidValue;
   const dim3 grid(nlat * P);
-  // tiles: threads per work-group, 1024 then 512 at L = 32; 768 at L = 48
-#define SG_SPLIT(l, p, t, nt)                                                              \
+  // tiles: threads per work-group, 1024 then 512 at L = 32; 768 at L = 48.
+  // The halo words are fetched before the swap (every wave polls) at L = 48,
+  // after it (one wave polls) at L = 32: 409-413 vs 432-437 us per 16-sweep
+  // launch at L = 48, 176-180 vs 172-174 us at L = 32 (64 lattices)
+#define SG_SPLIT(l, p, t, nt, hf)                                                          \
   if (L == l && P == p && tile == t) {                                                    \
     if (field)                                                                             \
-      hipLaunchKernelGGL((k_split<l, p, nt, true>), grid, dim3(nt), 0, st, jt, tasks, ticket,    \
-                         nsweeps, want_energy);                                            \
+      hipLaunchKernelGGL((k_split<l, p, nt, hf, true>), grid, dim3(nt), 0, st, jt, tasks,  \
+                         ticket, nsweeps, want_energy);                                    \
     else                                                                                   \
-      hipLaunchKernelGGL((k_split<l, p, nt, false>), grid, dim3(nt), 0, st, jt, tasks, ticket,   \
-                         nsweeps, want_energy);                                            \
+      hipLaunchKernelGGL((k_split<l, p, nt, hf, false>), grid, dim3(nt), 0, st, jt, tasks, \
+                         ticket, nsweeps, want_energy);                                    \
     return hipGetLastError();                                                              \
   }
-  SG_SPLIT(32, 4, 0, 1024)
-  SG_SPLIT(32, 4, 1, 512)
-  SG_SPLIT(48, 4, 0, 768)
+  SG_SPLIT(32, 4, 0, 1024, false)
+  SG_SPLIT(32, 4, 1, 512, false)
+  SG_SPLIT(48, 4, 0, 768, true)
 #undef SG_SPLIT
   return hipErrorInvalidValue;
 }
