@@ -2138,7 +2138,8 @@ __global__ __launch_bounds__(256) void k_pt_chunk(const JobPtrs *__restrict__ jt
 // there.  Thread t owns QPT = D L^2 / 2 NT quads q = t + m NT (m = 0 ..
 // QPT-1), numbered through the planes in the order 0, D - 1, 1, 2, ..., D -
 // 2, so that quads m < NB = L^2 / NT lie on the two boundary planes of the
-// halves.  Each quad's column and plane are packed in one VGPR.
+// halves.  Each quad's word offset, site parity and row-wrap flags are packed
+// in one VGPR, from which a phase derives the quad's neighbour offsets.
 //
 // A phase updates one colour c.  Its words stay in VGPRs; LDS holds a copy
 // of the other colour: the slab pair's own planes plus a halo plane below
@@ -2148,7 +2149,10 @@ __global__ __launch_bounds__(256) void k_pt_chunk(const JobPtrs *__restrict__ jt
 // the inner quads are updated while the planes travel.  Once every wave has
 // finished reading, the new colour-c words replace the copy (whose colour
 // 1 - c words go back to the VGPRs for the next phase), the work-group waits
-// for its two neighbours' flags and copies their planes into the halo rows.
+// for its two neighbours' flags and copies their planes into the halo rows:
+// with HALO_FIRST every wave polls and requests its halo words before the
+// swap, so they are in flight during it; otherwise thread 0 polls during the
+// swap and the copies follow a barrier.
 // Exchange data and flags go through device-coherent accesses (relaxed
 // agent-scope atomics, which pass the per-XCD L2s to the memory side); each
 // thread waits for its plane stores to complete before the barrier that
