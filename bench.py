@@ -40,6 +40,13 @@ import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "spin-glass_amd"))
+# The batch engine gives every (L, field) class its own stream; with HIP's
+# default of four hardware queues per process the five classes of the
+# serving workload share queues, and a class's launches wait behind
+# another's.  At least eight queues (set before the runtime starts) keep them
+# apart.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 import numpy as np  # noqa: E402
 

@@ -255,10 +255,12 @@ const char *sg_probe_last_error(void);
  * process (Program/src/main.cu:8-84).  An engine holds many jobs (each a
  * simulation as above, of any L, ladder, seeds and sweep count) on one
  * device, their state resident in a device arena between steps.  Jobs are
- * admitted in submission order while their footprint fits the arena; each
- * step runs slice_pt_steps PT intervals of every running job, all jobs of one
- * (L, field) class in one sweep-kernel launch per interval (SG_FORM_BATCHED)
- * or one launch per job (SG_FORM_PLAIN).  A job's results are bit-identical
+ * admitted in submission order while their footprint fits the arena.  The
+ * jobs of one (L, field) class share a stream and, per PT interval, one
+ * sweep-kernel launch (SG_FORM_BATCHED) or one launch per job
+ * (SG_FORM_PLAIN); each step queues slice_pt_steps PT intervals of every
+ * class whose stream holds fewer than max_inflight unfinished slices, so
+ * classes of short intervals run ahead of slower ones.  A job's results are bit-identical
  * to those of the same parameters run alone through sg_run: every random
  * number is keyed by the job's own seeds and counters.  One host thread per
  * engine. */
@@ -270,7 +272,7 @@ typedef struct sg_batch_params {
   int32_t slice_pt_steps;  /* PT intervals per sg_step (>= 1) */
   int32_t form;            /* SG_FORM_BATCHED or SG_FORM_PLAIN, as above */
   int64_t arena_bytes;     /* device memory for job state */
-  int32_t max_inflight;    /* sg_step calls queued on the device ahead of the host (>= 1) */
+  int32_t max_inflight;    /* slices queued per class on the device ahead of the host (>= 1) */
   int32_t pad;
 } sg_batch_params;
 
@@ -289,8 +291,9 @@ int64_t sg_job_footprint(const sg_params *p);
  * p->form and p->device are ignored.  SG_ERR_NOMEM when the footprint
  * exceeds the whole arena.  *id >= 0 names the job from then on. */
 sg_status sg_submit(sg_batch *b, const sg_params *p, int64_t n_sweeps, int64_t *id);
-/* one step: retire finished jobs, admit queued ones, run one slice of every
- * running job.  Returns the counts after the step (either may be NULL). */
+/* one step: retire finished jobs, admit queued ones, queue one slice of
+ * every class with room (waiting for a slot when none has).  Returns the
+ * counts after the step (either may be NULL). */
 sg_status sg_step(sg_batch *b, int32_t *n_running, int32_t *n_queued);
 /* steps until every submitted job is done */
 sg_status sg_drain(sg_batch *b);

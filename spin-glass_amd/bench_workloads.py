@@ -170,14 +170,19 @@ def pretune(field=False, max_lattices=8192):
             n *= 2
 
 
+# serving engine: PT intervals per step and per class slice, slices queued per
+# class ahead of the host
+SERVE_SLICE, SERVE_INFLIGHT = 4, 4
+
+
 def serve_run(n_jobs, rate, seed=0x5EED0005):
     """One run of the native serving loop over the seeded job stream: jobs
     arrive (Poisson, `rate` per second), are admitted by footprint, run in
     continuous batches and are collected; returns the latency percentiles and
     throughput."""
     jobs, arrivals = serve_jobs(n_jobs, rate, seed)
-    b = sg.Batch(pt_every=PT_EVERY, slice_pt_steps=2, form=sg.SG_FORM_BATCHED,
-                 arena_bytes=16 << 30)
+    b = sg.Batch(pt_every=PT_EVERY, slice_pt_steps=SERVE_SLICE, form=sg.SG_FORM_BATCHED,
+                 arena_bytes=16 << 30, max_inflight=SERVE_INFLIGHT)
     w = [b.submit(sg.Job(L=L, n_sweeps=64, T_range=(0.2, 2.0, 16))) for L in L_CHOICES]
     b.drain()
     for i in w:
@@ -199,8 +204,8 @@ def serve_run(n_jobs, rate, seed=0x5EED0005):
     # alone on an idle engine, submission to result
     big = max(range(n_jobs), key=lambda i: jobs[i].n_sweeps * jobs[i].L ** 3)
     j = jobs[big]
-    b = sg.Batch(pt_every=PT_EVERY, slice_pt_steps=2, form=sg.SG_FORM_BATCHED,
-                 arena_bytes=16 << 30)
+    b = sg.Batch(pt_every=PT_EVERY, slice_pt_steps=SERVE_SLICE, form=sg.SG_FORM_BATCHED,
+                 arena_bytes=16 << 30, max_inflight=SERVE_INFLIGHT)
     i = b.submit(sg.Job(L=j.L, n_sweeps=j.n_sweeps, n_samples=64, T_range=j.T_range,
                         coupling_seed=j.coupling_seed, spin_seed=j.spin_seed,
                         meas_every=1024))

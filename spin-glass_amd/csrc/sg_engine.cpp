@@ -152,6 +152,7 @@ struct Class {
   DevList<TaskDesc> ptc[2];     // streamed: (pair, chunk) tasks of the PT launch
   hipEvent_t up_ev = nullptr;   // the last upload from the staging
   bool uploaded = false;
+  std::deque<hipEvent_t> inflight;   // one event after each slice queued on st
   Grave grave;                  // outgrown table arrays
   int64_t grown = 0;            // table growths so far
   // Batched intervals captured as hipGraphs: one per (first parity list,
@@ -212,20 +213,17 @@ struct sg_batch {
   std::map<int64_t, std::unique_ptr<Job>> jobs;
   std::deque<Job *> queue;            // submitted, not admitted
   std::vector<Class> classes;
-  std::deque<std::vector<hipEvent_t>> inflight;  // per step: one event per busy class
   std::vector<hipEvent_t> ev_pool;
   int64_t next_id = 0, steps = 0;
   int64_t iv_direct = 0, iv_graphed = 0, graphs_built = 0;   // batched class intervals
   double flips = 0;
 
   ~sg_batch() {
-    for (auto &s : inflight)
-      for (auto e : s) (void)hipEventSynchronize(e);
     for (auto &c : classes)
       if (c.st) (void)hipStreamSynchronize(c.st);
     jobs.clear();  // job handles free into the arena before it goes
-    for (auto &s : inflight)
-      for (auto e : s) (void)hipEventDestroy(e);
+    for (auto &c : classes)
+      for (auto e : c.inflight) (void)hipEventDestroy(e);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     for (auto &c : classes) c.free_all();
   }
@@ -631,28 +629,55 @@ struct sg_batch {
     }
   }
 
+  // One step: retire finished jobs, admit queued ones, and queue one slice
+  // (slice_pt_steps PT intervals) of every class whose stream holds fewer
+  // than max_inflight unfinished slices.  Classes are paced by their own
+  // queues, so a class of short intervals runs ahead of a slower one instead
+  // of waiting for it; when every busy class is full the host waits for the
+  // first free slot.
   void step() {
     retire(false);
     admit();
-    std::vector<hipEvent_t> evs;
+    const int cap = std::max(bp.max_inflight, 1);
+    bool ran = false;
     for (auto &c : classes) {
-      if (c.jobs.empty()) continue;
+      reap_slices(c);
+      if (c.jobs.empty() || (int)c.inflight.size() >= cap) continue;
       run_class(c);
       hipEvent_t e = take_event();
       HIPCHK(hipEventRecord(e, c.st));
-      evs.push_back(e);
+      c.inflight.push_back(e);
+      ran = true;
     }
     ++steps;
-    if (!evs.empty()) inflight.push_back(std::move(evs));
-    // keep at most max_inflight steps queued on the device
-    while ((int)inflight.size() > std::max(bp.max_inflight, 1)) {
-      for (auto e : inflight.front()) {
-        HIPCHK(hipEventSynchronize(e));
-        ev_pool.push_back(e);
-      }
-      inflight.pop_front();
-    }
+    if (!ran) wait_for_slot(cap);
     retire(false);
+  }
+
+  // the finished slices at the head of the class's queue
+  void reap_slices(Class &c) {
+    while (!c.inflight.empty()) {
+      const hipError_t q = hipEventQuery(c.inflight.front());
+      if (q == hipErrorNotReady) break;
+      HIPCHK(q);
+      ev_pool.push_back(c.inflight.front());
+      c.inflight.pop_front();
+    }
+  }
+
+  // until a class with jobs has room for another slice (none: at once)
+  void wait_for_slot(int cap) {
+    for (;;) {
+      bool busy_class = false;
+      for (auto &c : classes) {
+        reap_slices(c);
+        if (c.jobs.empty()) continue;
+        busy_class = true;
+        if ((int)c.inflight.size() < cap) return;
+      }
+      if (!busy_class) return;
+      std::this_thread::yield();
+    }
   }
 
   int n_running() const {
