@@ -314,6 +314,35 @@ def test_second_wave_of_jobs_allocates_nothing():
 
 
 @pytest.mark.gpu
+def test_classes_are_paced_by_their_own_queues():
+    """An L = 24 job and an L = 8 job of the same sweep count submitted
+    together: each class's stream is refilled as its own slices finish, so
+    the L = 8 job (intervals of a few tens of microseconds) is done long
+    before the L = 24 one (about 150 us per interval) instead of moving at
+    its pace; both results equal the jobs run alone."""
+    def jobs(n):
+        return [sg.Job(L=24, n_sweeps=n, n_samples=64, T_range=(0.3, 2.0, 32),
+                       coupling_seed=91, spin_seed=92),
+                sg.Job(L=8, n_sweeps=n, n_samples=64, T_range=(0.3, 2.0, 16),
+                       coupling_seed=93, spin_seed=94)]
+
+    def run(n):
+        b = sg.Batch(pt_every=PT, slice_pt_steps=4, max_inflight=4, arena_bytes=1 << 30)
+        ids = [b.submit(j) for j in jobs(n)]
+        b.drain()
+        took = [b.times(i)[2] - b.times(i)[0] for i in ids]
+        got = [b.result(i) for i in ids]
+        b.close()
+        return took, got
+
+    run(256)   # first launches of both classes' kernels and graphs
+    took, got = run(4096)
+    print(f"L=24 {took[0]:.4f} s, L=8 {took[1]:.4f} s", flush=True)
+    assert took[1] < 0.7 * took[0]     # in step with the L = 24 class: about 1
+    _check_same(got, [_alone(j) for j in jobs(4096)])
+
+
+@pytest.mark.gpu
 def test_serving_workload_jobs_equal_their_alone_runs():
     """The serving configuration, scaled to test time: 24 jobs arriving as a
     Poisson stream at 200/s (seed 0x5EED0005), L from {8..24}, 16-32
