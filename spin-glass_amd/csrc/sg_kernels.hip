@@ -2264,6 +2264,7 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
   // (y + zl) & 1; bits 18, 19: xh = 0, xh = L/2 - 1; bits 20, 21: y = 0,
   // y = L - 1
   static_assert(8 * G::QW < (1 << 17), "descriptor byte offsets");
+  const int wave_i = __builtin_amdgcn_readfirstlane(tid) & ~63;   // the wave's first thread
   uint32_t qd[G::QPT];
 #pragma unroll
   for (int m = 0; m < G::QPT; ++m) {
@@ -2416,7 +2417,6 @@ __global__ __launch_bounds__(NT) void k_split(const JobPtrs *__restrict__ jt,
       // halo word k of this thread (fetched before the swap): i = tid + k NT;
       // the 64 words of a wave lie in one row (PL is a multiple of 64), so
       // the row arithmetic is the wave's (scalar) and a lane adds its index
-      const int wave_i = __builtin_amdgcn_readfirstlane(tid) & ~63;
       auto halo_word = [&](int k) {
         const int i0 = wave_i + k * NT, hs = i0 / (2 * G::PL), top = (i0 / G::PL) & 1;
         const uint64_t *src = top ? xab + ((hs ^ (w == P - 1 ? 1 : 0)) * 2 + 0) * G::PL
