@@ -20,7 +20,7 @@ W_ALG = 1.22
 # Memory-side bytes per launch of the large-lattice kernels at 64 lattices (FETCH_SIZE +
 # WRITE_SIZE, scaled by a copy of known size), keyed by (L, shape, tile): the split kernel per
 # 16-sweep launch (most of it the boundary-plane exchange); profiles/k_split_L32_L48_64lattices.txt
-TRAFFIC_LARGE = {(32, sg.SG_KERNEL_SPLIT, 0): 310.4e6, (48, sg.SG_KERNEL_SPLIT, 0): 775.9e6}
+TRAFFIC_LARGE = {(32, sg.SG_KERNEL_SPLIT, 0): 286.8e6, (48, sg.SG_KERNEL_SPLIT, 0): 771.3e6}
 LARGE_KERNELS = {(32, sg.SG_KERNEL_SPLIT, 0): "k_split<32,4,1024,false,false> (16 sweeps per launch)",
                  (32, sg.SG_KERNEL_SPLIT, 1): "k_split<32,4,512,false,false> (16 sweeps per launch)",
                  (48, sg.SG_KERNEL_SPLIT, 0): "k_split<48,4,768,true,false> (16 sweeps per launch)",
