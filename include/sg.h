@@ -80,11 +80,12 @@ enum {
   SG_KERNEL_RESIDENT_SINGLE = 3, /* one lattice per work-group (twice the work-groups of
                                    RESIDENT), PT in a launch of its own; the same as
                                    RESIDENT at L = 22..26, whose kernel holds one lattice */
-  SG_KERNEL_SPLIT = 4     /* L = 32 and 48 (<= 64 lattices per launch): a lattice
-                             held on chip by 4 work-groups for a PT interval,
-                             exchanging boundary planes (all work-groups of a
-                             lattice run at once); PT in a launch of its own.  Not
-                             used by the batch engine */
+  SG_KERNEL_SPLIT = 4     /* L = 32 and 48: a lattice held on chip by 4
+                             work-groups for a PT interval, exchanging boundary
+                             planes (all work-groups of a launch run at once: 64
+                             lattices a launch, more as consecutive launches); PT
+                             in a launch of its own.  Not used by the batch
+                             engine */
 };
 
 /* Per-measurement callback: rec holds n_groups*n_temps*64 records of nfield

@@ -917,11 +917,16 @@ struct sg_handle {
       const int pb = per_group ? ptc_off[par][g + 1] : ptc_off[par][G];
       if (do_pt) HIPCHK(launch_pt_chunks(d_self, d_ptc[par] + pa, pb - pa, st));
       const int a = per_group ? g * per : 0, n = per_group ? per : (int)n_lat();
-      const double fl = (double)seg * N * 64.0 * n;
-      timed(0, fl, [&] {
-        HIPCHK(launch_split(L, field, d_self, d_split + a, n, split_P, d_ticket, seg, want_energy,
-                            tile, st));
-      });
+      // at most split_batch(P) lattices a launch (all their work-groups on
+      // the CUs at once); more go out as consecutive launches
+      for (int b = 0; b < n; b += split_batch(split_P)) {
+        const int nb = std::min(n - b, split_batch(split_P));
+        const double fl = (double)seg * N * 64.0 * nb;
+        timed(0, fl, [&] {
+          HIPCHK(launch_split(L, field, d_self, d_split + a + b, nb, split_P, d_ticket, seg,
+                              want_energy, tile, st));
+        });
+      }
     }
     HIPCHK(launch_advance(d_self, 1, seg, st));
   }

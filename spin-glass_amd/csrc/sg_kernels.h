@@ -72,9 +72,11 @@ hipError_t launch_slab(int L, bool field, const JobPtrs *jt, const TaskDesc *tas
 // for a whole PT interval, exchanging boundary planes through jt's xbuf /
 // xflag (k_split).  split_slabs: the P used for n_lattices lattices of size
 // L (0: no split kernel for this L); P > 1 needs all n_lattices * P
-// work-groups resident at once (one per CU), so callers keep n_lattices * P
-// <= 256 and never run two split launches at the same time.
+// work-groups of a launch resident at once (one per CU), so a launch takes
+// at most split_batch(P) lattices (more run as consecutive launches) and
+// callers never run two split launches at the same time.
 int split_slabs(int L, int n_lattices);
+constexpr int split_batch(int P) { return 256 / P; }
 constexpr int kSplitTicketWords = 8;   // one ticket per slab pair (P <= 7) and a done count
 // Tasks: lat[0] = lattice descriptor, one per lattice; ticket:
 // kSplitTicketWords zeroed uint32 counters kept for this task list (the

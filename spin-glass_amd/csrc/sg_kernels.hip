@@ -2768,8 +2768,8 @@ hipError_t launch_advance(const JobPtrs *jt, int njobs, int nsweeps, hipStream_t
 // The split kernels, in a translation unit of their own (sg_kernels_split.hip).
 int split_slabs(int L, int n_lattices) {
   // four slab pairs per lattice (4 + 4 planes each at L = 32, 6 + 6 at
-  // L = 48), one work-group per CU: up to 64 lattices
-  return (L == 32 || L == 48) && n_lattices * 4 <= 256 ? 4 : 0;
+  // L = 48), one work-group per CU: 64 lattices per launch
+  return (L == 32 || L == 48) && n_lattices >= 1 ? 4 : 0;
 }
 
 int split_tiles(int L) { return L == 32 ? 2 : L == 48 ? 1 : 0; }

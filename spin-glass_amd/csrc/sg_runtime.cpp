@@ -181,7 +181,7 @@ sg_status sg_create(const sg_params *p, sg_handle **out) {
     const int64_t per = (int64_t)p->n_replicas * p->n_temps;
     const int64_t n = p->form == SG_FORM_PLAIN ? per : per * ((p->n_samples + 63) / 64);
     if (n > 4096 || split_slabs(p->L, (int)n) == 0)
-      return bad("no split kernel for this L and size (L = 32, up to 64 lattices per launch)");
+      return bad("no split kernel for this L and size (L = 32 or 48)");
   }
   sg_handle *h = new (std::nothrow) sg_handle();
   if (!h) return bad("host allocation failed");

@@ -36,8 +36,7 @@ def test_create_rejects_bad_parameters_without_device():
                dict(n_replicas=0), dict(n_replicas=9), dict(betas=[0.5, -1.0]),
                dict(betas=[1.0] * 4097), dict(kernel=5), dict(form=2), dict(tile=-1),
                dict(kernel=sg.SG_KERNEL_SPLIT),                      # no split kernel at L = 8
-               dict(L=32, n_samples=128, betas=[1.0] * 24, kernel=sg.SG_KERNEL_SPLIT,
-                    form=sg.SG_FORM_BATCHED)):                       # 96 lattices: too many
+               dict(L=40, kernel=sg.SG_KERNEL_SPLIT)):               # nor at L = 40
         args = dict(L=8, n_samples=64, betas=[1.0, 0.5])
         args.update(kw)
         try:

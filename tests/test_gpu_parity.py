@@ -144,6 +144,26 @@ def test_split_kernel_second_tile_matches_oracle(form, H):
         np.testing.assert_array_equal(rec, orc.meas(i))
 
 
+@pytest.mark.parametrize("L,T,H,form", [(32, 24, 0.0, sg.SG_FORM_BATCHED),
+                                         (48, 20, 0.5, sg.SG_FORM_BATCHED),
+                                         (32, 40, 0.5, sg.SG_FORM_PLAIN)])
+def test_split_kernel_beyond_one_launch_matches_oracle(L, T, H, form):
+    """More lattices than one split launch holds (64: one work-group per CU
+    for each of their four slab pairs): 128 samples, so 96 and 80 lattices in
+    the batched form and 80 per group in the plain form, run as consecutive
+    launches of at most 64."""
+    betas = geometric_betas(0.3, 2.0, T)
+    sim, orc = _pair(L, 128, 2, betas, H=H, pt_every=4, meas_every=8, form=form,
+                     kernel=sg.SG_KERNEL_SPLIT)
+    assert sim.shape[0] == sg.SG_KERNEL_SPLIT
+    for n in (4, 9):
+        sim.run(n)
+        orc.run(n)
+        _assert_same(sim, orc)
+    for i, (_, rec) in enumerate(sim.records):
+        np.testing.assert_array_equal(rec, orc.meas(i))
+
+
 def test_missing_tile_is_rejected():
     betas = geometric_betas(0.3, 2.0, 4)
     for kw in (dict(L=48, kernel=sg.SG_KERNEL_SPLIT, tile=2),   # one tile at L = 48
