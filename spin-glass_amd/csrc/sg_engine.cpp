@@ -7,7 +7,9 @@
 // class share a HIP stream, and in the batched form one sweep-kernel launch
 // per PT interval covers every task of every running job of the class
 // (TaskDesc.job indexes the class's JobPtrs table).  Classes run on separate
-// streams, so the device overlaps them.  A class's work-group shape comes
+// streams, so the device overlaps them, and each is refilled as its own
+// queued slices finish, so a class of short intervals is not held to the
+// pace of a slower one.  A class's work-group shape comes
 // from the tuner's cache (or the rule its measurements support): no step runs
 // the tuner.  Because each job's random numbers
 // are keyed by its own seeds and counters, the mix never changes a result.
