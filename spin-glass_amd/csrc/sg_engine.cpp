@@ -738,10 +738,12 @@ void sg_batch_params_default(sg_batch_params *bp) {
   std::memset(bp, 0, sizeof(*bp));
   bp->device = 0;
   bp->pt_every = 16;
-  bp->slice_pt_steps = 1;
+  // slices of four intervals with four queued per class: the best of (2, 2),
+  // (2, 8), (4, 4) and (4, 16) on the serving workload
+  bp->slice_pt_steps = 4;
   bp->form = SG_FORM_BATCHED;
   bp->arena_bytes = (int64_t)4 << 30;
-  bp->max_inflight = 2;
+  bp->max_inflight = 4;
 }
 
 sg_status sg_batch_create(const sg_batch_params *bp, sg_batch **out) {

@@ -450,8 +450,8 @@ class Batch:
     by footprint, one sweep-kernel launch per (L, field) class and PT
     interval (form SG_FORM_BATCHED) or per job (SG_FORM_PLAIN)."""
 
-    def __init__(self, pt_every=16, slice_pt_steps=1, form=SG_FORM_BATCHED,
-                 arena_bytes=4 << 30, max_inflight=2, device=0):
+    def __init__(self, pt_every=16, slice_pt_steps=4, form=SG_FORM_BATCHED,
+                 arena_bytes=4 << 30, max_inflight=4, device=0):
         lib = load_library()
         bp = _BatchParams()
         lib.sg_batch_params_default(ctypes.byref(bp))
