@@ -67,7 +67,7 @@ HBM_PEAK_GBS = 8000.0
 # 16 sweeps): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of that kernel, each
 # scaled by what the counter reads for a device copy of known size (the
 # --copy-probe run); profiles/k_block_L16_batched_2048_traffic.txt.
-TRAFFIC_L16 = 109.4e6
+TRAFFIC_L16 = 109.3e6
 # VALU wave-instructions per k_block<16,false> launch of the l16 workload
 # (SQ_INSTS_VALU, profiles/k_block_L16_batched_2048_counters.txt): x 64 lanes
 # over the launch duration gives the issued lane-ops/s.
